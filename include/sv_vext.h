@@ -1,0 +1,141 @@
+/*
+ * libsv_vext.so - MI355X (gfx950) guest-framework Lennard-Jones Vext.
+ *
+ * C ABI only: plain pointers, sizes and opaque handles.  All pointers are
+ * host pointers unless a function says otherwise.  Every function returns 0
+ * on success and a nonzero code on failure; sv_last_error() returns a
+ * thread-local description of the last failure.
+ *
+ * Return codes shared by every entry point:
+ *   0  success
+ *   1  invalid size/argument      (reference native/vext/kernel.cpp:11-12)
+ *   2  non-monotone CSR starts    (kernel.cpp:14-15)
+ *   3  neighbor index out of range (kernel.cpp:24-25)
+ *   4  non-finite energy          (kernel.cpp:39-40)
+ *   5  HIP runtime error (no device, launch failure, ...)
+ *   6  device memory exhausted
+ */
+#ifndef SV_VEXT_H
+#define SV_VEXT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in entry point.                                                      */
+/* Replaces evaluate_reused_neighbors of the reference CPU kernel            */
+/* (native/vext/kernel.cpp:5-10), loaded by NativeReusePotential through     */
+/* ctypes (src/string_vext/native.py:18-22) and selectable with              */
+/* `string-vext --kernel`.  Same arguments, same return codes:               */
+/*   centers [n_centers][3] Cartesian, axes [n_axes][3] unit directors,      */
+/*   images [n_images][3] Cartesian, offsets[2] site offsets along the axis, */
+/*   coeff12/coeff6/shift/floor2 [n_images], CSR starts [n_centers+1] and    */
+/*   indices [starts[n_centers]] of candidate images per center,             */
+/*   output [n_centers][n_axes] (FP64).                                       */
+/* Evaluated on the GPU in FP64 with the reference's operation order, so a   */
+/* center's energies equal the reference CPU results on the same CSR.        */
+/* Synchronous and thread-safe (calls are serialized).  On error, rows of    */
+/* centers before the failing one hold their energies.                       */
+/* ------------------------------------------------------------------------ */
+int evaluate_reused_neighbors(int n_centers, int n_axes, int n_images, double cutoff2,
+                              const double *centers, const double *axes, const double *images,
+                              const double *offsets, const double *coeff12,
+                              const double *coeff6, const double *shift, const double *floor2,
+                              const int64_t *starts, const int32_t *indices, double *output);
+
+/* ------------------------------------------------------------------------ */
+/* Library information                                                       */
+/* ------------------------------------------------------------------------ */
+const char *sv_last_error(void);
+/* ABI version of this header (increases when entry points are added). */
+int sv_abi_version(void);
+/* Number of visible HIP devices (0 when none). */
+int sv_device_count(void);
+
+/* ------------------------------------------------------------------------ */
+/* Framework plan: one framework, one two-site guest, one cutoff.            */
+/* Replaces the setup of ReusePotential / TriclinicSitePotential             */
+/* (src/string_vext/neighbor.py:9-30, physics.py:42-57): cell matrix rows    */
+/* are the lattice vectors (cell.py:24-26), framework coordinates are        */
+/* fractional, Lorentz-Berthelot mixing with the guest, shifted cutoff at rc,*/
+/* contact floor (0.1 sigma)^2.  Device tables are built once per plan.      */
+/*   cell[9]            row-major 3x3, det > 0                                */
+/*   frac [n_atoms][3], sigma/epsilon [n_atoms] (> 0)                         */
+/*   guest_sigma/epsilon  the two identical guest sites                      */
+/*   offsets[2]         signed site offsets along the director               */
+/*                      (physics.py:88-90)                                   */
+/*   near_energy_K      bound on a single FP32 pair term in the mixed        */
+/*                      precision kernel; closer atoms are evaluated in FP64. */
+/*                      <= 0 disables the FP64 near path.                    */
+/* ------------------------------------------------------------------------ */
+int sv_plan_create(const double *cell, int n_atoms, const double *frac, const double *sigma,
+                   const double *epsilon, double guest_sigma, double guest_epsilon,
+                   const double *offsets, double cutoff, double near_energy_K, void **plan);
+int sv_plan_destroy(void *plan);
+/* Number of distinct framework LJ types of the plan. */
+int sv_plan_info(void *plan, int *n_types, double *keep_radius);
+
+/* ------------------------------------------------------------------------ */
+/* Pose sets: every (center, axis) pair of n_centers fractional centers and  */
+/* n_axes unit directors (the product evaluated by NativeReusePotential.     */
+/* evaluate, native.py:26-68).  Centers are wrapped into [0,1) like          */
+/* np.mod(centers, 1) (native.py:35).  sv_poses_grid uses the regular pose   */
+/* grid centers (i+0.5)/size on an ij-indexed meshgrid (orientation.py:7-16).*/
+/* ------------------------------------------------------------------------ */
+int sv_poses_create(void *plan, int n_centers, const double *centers_frac, int n_axes,
+                    const double *axes, void **poses);
+int sv_poses_grid(void *plan, int size, int n_axes, const double *axes, void **poses);
+int sv_poses_destroy(void *poses);
+
+/* ------------------------------------------------------------------------ */
+/* Whole-field evaluation, host outputs (synchronous).                       */
+/*   precision   0 = FP64 (reference arithmetic), 1 = mixed FP32 pair math   */
+/*               with FP64 accumulation and FP64 near-contact terms          */
+/*   energy      [n_centers][n_axes] FP64   (reference kernel layout) or NULL */
+/*   orientation [n_axes][n_centers] FP32   (vext.py:31 orientation_K) or NULL*/
+/*   marginal    [n_centers] FP32  -T(logsumexp(-E/T) - ln K) (physics.py:126)*/
+/*   grid        [n_centers] FP32  asinh(marginal/1000)        (vext.py:40)   */
+/*   site        [n_centers] FP32  single-site field            (vext.py:27-29)*/
+/*   temperature  needed when marginal or grid is requested                  */
+/* Returns 4 when any requested output is non-finite.                        */
+/* ------------------------------------------------------------------------ */
+int sv_field_run(void *plan, void *poses, int precision, double temperature, double *energy,
+                 float *orientation, float *marginal, float *grid, float *site);
+
+/* Exact number of in-cutoff (site, image) pairs of the pose set: the        */
+/* "pair-eval" count of the throughput metric (FP64 cutoff test).            */
+int sv_field_census(void *plan, void *poses, unsigned long long *pairs);
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many (plan, poses) fields evaluated by one set of launches.      */
+/* Outputs are DEVICE pointers (arrays of n_jobs pointers, entries may be    */
+/* NULL).  sv_batch_run is asynchronous on the batch's stream unless         */
+/* `sync` is nonzero; sv_batch_sync waits and reports non-finite outputs.    */
+/* With time_kernels nonzero, HIP events bracket each phase on the stream    */
+/* and sv_batch_timings returns the last run's milliseconds per phase:       */
+/*   [0] list count, [1] scan, [2] list fill, [3] field, [4] marginal,        */
+/*   [5] site field, [6] whole run.                                           */
+/* ------------------------------------------------------------------------ */
+int sv_batch_create(int n_jobs, void *const *plans, void *const *poses, int precision,
+                    const double *temperatures, double *const *energy,
+                    float *const *orientation, float *const *marginal, float *const *grid,
+                    float *const *site, void **batch);
+int sv_batch_run(void *batch, int time_kernels, int sync);
+int sv_batch_sync(void *batch);
+int sv_batch_timings(void *batch, double *ms7);
+int sv_batch_census(void *batch, unsigned long long *pairs_per_job);
+int sv_batch_destroy(void *batch);
+
+/* Device memory helpers for callers without their own allocator. */
+int sv_device_malloc(void **ptr, uint64_t bytes);
+int sv_device_free(void *ptr);
+int sv_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+int sv_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SV_VEXT_H */

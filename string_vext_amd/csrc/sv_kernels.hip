@@ -1,0 +1,711 @@
+// gfx950 kernels for the guest-framework Lennard-Jones Vext path.
+//
+// Physics (restated from native/vext/kernel.cpp:13-44 and
+// src/string_vext/neighbor.py:17-28 of the reference):
+//   E(c, u) = sum_{s in {0,1}} sum_{images j, |d|^2 < rc^2} c12_j i6^2 - c6_j i6 + shift_j
+//   d = c + o_s u - img_j,  i6 = (1 / max(|d|^2, floor2_j))^3.
+//
+// Work decomposition (one wave = one pose center x 64 orientations):
+//   * pose centers are binned into spatial tiles; every tile owns a candidate
+//     list of framework images within (tile radius + rc + max|o|), grouped by
+//     LJ type and stored tile-relative (float4 + an FP64 copy);
+//   * a wave streams its tile's list 64 entries at a time, keeps the entries
+//     within rc + max|o| of its own center (ballot + mbcnt compaction into a
+//     per-wave LDS buffer), then every lane (one orientation) sums both guest
+//     sites against the compacted atoms, read from LDS by broadcast.
+//   * with r = img - c (uniform over the wave) and t = r.u (per lane), the
+//     two site distances are |r|^2 + o_s^2 - 2 o_s t: one dot product serves
+//     both sites.
+//
+// Mixed precision: the pair arithmetic is FP32 with the distance clamped to
+// [floor2, rc^2] by one v_med3_f32.  Because the shifted potential vanishes at
+// rc, a clamped out-of-range pair contributes exactly -shift, which the
+// per-type count of kept pairs adds back in FP64; no cutoff branch or select
+// is needed and a pair on the wrong side of rc by FP32 rounding changes the
+// energy by ~1e-8 K.  Per-chunk FP32 partials are flushed into an FP64
+// accumulator.  Atoms closer to the center than a per-type near radius (where
+// a single pair term can exceed the configured bound, default 50 K) are
+// evaluated in FP64 with the reference formula, so overlap poses keep FP64
+// relative accuracy.
+#include "sv_internal.h"
+#include <climits>
+
+namespace sv {
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Number of set bits of m in lanes below the calling lane.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Blocks are dispatched round-robin over the 8 XCDs.  Map hardware block b to
+// a logical block so that consecutive logical blocks (same tile, same list)
+// run on the same XCD and share its L2.
+__device__ __forceinline__ long long xcd_block(long long b, long long nb) {
+  const long long q = nb / 8, r = nb % 8;
+  const long long x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ int find_job(const long long* __restrict__ job_work, int n_jobs,
+                                        long long w) {
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (job_work[mid] <= w) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_exclusive_scan(int v, int lane, int* total) {
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  *total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+// Range of lattice shifts n_k whose image f + n lies within `radius` of the
+// fractional point f0: |(f - f0 + n) H| <= radius implies
+// |f_k - f0_k + n_k| <= radius * |column k of H^-1|.
+struct ShiftBox {
+  int lo[3], hi[3];
+};
+__device__ __forceinline__ ShiftBox shift_box(const double d[3], const double hn[3], double radius) {
+  ShiftBox b;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double h = radius * hn[k];
+    b.lo[k] = (int)ceil(-d[k] - h);
+    b.hi[k] = (int)floor(-d[k] + h);
+  }
+  return b;
+}
+
+// Cartesian offset of lattice image (d + n) and whether it lies within r2.
+// Written with explicit fma so the count and fill kernels (and both passes of
+// the fill kernel) round identically and agree on every borderline image.
+__device__ __forceinline__ bool image_inside(const double d[3], int na, int nb, int nc,
+                                             const double* __restrict__ H, double r2, double* x,
+                                             double* y, double* z) {
+  const double ua = d[0] + na, ub = d[1] + nb, uc = d[2] + nc;
+  *x = fma(ua, H[0], fma(ub, H[3], uc * H[6]));
+  *y = fma(ua, H[1], fma(ub, H[4], uc * H[7]));
+  *z = fma(ua, H[2], fma(ub, H[5], uc * H[8]));
+  return fma(*x, *x, fma(*y, *y, *z * *z)) <= r2;
+}
+
+// ---------------------------------------------------------------------------
+// Tile candidate lists
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void tile_count_kernel(const Job* __restrict__ jobs,
+                                                        const Tile* __restrict__ tiles,
+                                                        const long long* __restrict__ seg_base,
+                                                        int* __restrict__ counts) {
+  const int g = blockIdx.x, lane = threadIdx.x;
+  const Tile& T = tiles[g];
+  const Job& J = jobs[T.job];
+  const double r2 = T.list_r * T.list_r;
+  for (int t = 0; t < J.n_types; ++t) {
+    int count = 0;
+    for (int a = J.type_start[t] + lane; a < J.type_start[t + 1]; a += 64) {
+      double d[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d[k] = J.frac[3 * a + k] - T.frac[k];
+      const ShiftBox b = shift_box(d, J.hn, T.list_r);
+      for (int na = b.lo[0]; na <= b.hi[0]; ++na)
+        for (int nb = b.lo[1]; nb <= b.hi[1]; ++nb)
+          for (int nc = b.lo[2]; nc <= b.hi[2]; ++nc) {
+            double x, y, z;
+            count += image_inside(d, na, nb, nc, J.H, r2, &x, &y, &z);
+          }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) count += __shfl_xor(count, o, 64);
+    if (lane == 0) counts[seg_base[g] + t] = count;
+  }
+}
+
+// Exclusive scan of counts[0..n) into offsets[0..n]; one block of 1024 threads.
+__global__ __launch_bounds__(1024) void scan_kernel(const int* __restrict__ counts, long long n,
+                                                    long long* __restrict__ offsets) {
+  __shared__ long long s_part[1024];
+  const int tid = threadIdx.x;
+  const long long per = (n + 1023) / 1024;
+  const long long beg = tid * per < n ? tid * per : n;
+  const long long end = beg + per < n ? beg + per : n;
+  long long sum = 0;
+  for (long long i = beg; i < end; ++i) sum += counts[i];
+  s_part[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const long long v = tid >= o ? s_part[tid - o] : 0;
+    __syncthreads();
+    s_part[tid] += v;
+    __syncthreads();
+  }
+  long long run = s_part[tid] - sum;
+  for (long long i = beg; i < end; ++i) {
+    offsets[i] = run;
+    run += counts[i];
+  }
+  if (tid == 1023) offsets[n] = s_part[1023];
+}
+
+__global__ __launch_bounds__(64) void tile_fill_kernel(const Job* __restrict__ jobs,
+                                                       const Tile* __restrict__ tiles,
+                                                       const long long* __restrict__ seg_base,
+                                                       const long long* __restrict__ offsets,
+                                                       float4* __restrict__ list32,
+                                                       double* __restrict__ list64) {
+  const int g = blockIdx.x, lane = threadIdx.x;
+  const Tile& T = tiles[g];
+  const Job& J = jobs[T.job];
+  const double r2 = T.list_r * T.list_r;
+  for (int t = 0; t < J.n_types; ++t) {
+    long long base = offsets[seg_base[g] + t];
+    const long long seg_end = offsets[seg_base[g] + t + 1];
+    const int a_end = J.type_start[t + 1];
+    for (int a0 = J.type_start[t]; a0 < a_end; a0 += 64) {
+      const int a = a0 + lane;
+      const bool ok = a < a_end;
+      double d[3] = {0.0, 0.0, 0.0};
+      ShiftBox b = {{0, 0, 0}, {-1, -1, -1}};
+      if (ok) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d[k] = J.frac[3 * a + k] - T.frac[k];
+        b = shift_box(d, J.hn, T.list_r);
+      }
+      // Pass 1: how many images this lane emits.
+      int mine = 0;
+      for (int na = b.lo[0]; na <= b.hi[0]; ++na)
+        for (int nb = b.lo[1]; nb <= b.hi[1]; ++nb)
+          for (int nc = b.lo[2]; nc <= b.hi[2]; ++nc) {
+            double x, y, z;
+            mine += image_inside(d, na, nb, nc, J.H, r2, &x, &y, &z);
+          }
+      int total;
+      long long slot = base + wave_exclusive_scan(mine, lane, &total);
+      // Pass 2: write them in lane order (deterministic list order), never
+      // past the end of this (tile, type) segment.
+      for (int na = b.lo[0]; na <= b.hi[0]; ++na)
+        for (int nb = b.lo[1]; nb <= b.hi[1]; ++nb)
+          for (int nc = b.lo[2]; nc <= b.hi[2]; ++nc) {
+            double x, y, z;
+            if (image_inside(d, na, nb, nc, J.H, r2, &x, &y, &z) && slot < seg_end) {
+              list32[slot] = make_float4((float)x, (float)y, (float)z, __int_as_float(a));
+              list64[3 * slot + 0] = x;
+              list64[3 * slot + 1] = y;
+              list64[3 * slot + 2] = z;
+              ++slot;
+            }
+          }
+      base += total;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Field kernels
+// ---------------------------------------------------------------------------
+struct WaveCtx {
+  const Job* J;
+  int slot;       // sorted center slot
+  int tile;       // job-local tile
+  int k;          // this lane's orientation
+  bool k_ok;
+};
+
+__device__ __forceinline__ bool setup_wave(const Job* __restrict__ jobs, int n_jobs,
+                                           const long long* __restrict__ job_work,
+                                           long long n_work, int wv, WaveCtx& c) {
+  const long long w = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
+  if (w >= n_work) return false;
+  const int j = find_job(job_work, n_jobs, w);
+  c.J = jobs + j;
+  const long long local = w - c.J->work_begin;
+  const int ng = c.J->n_groups;
+  c.slot = (int)(local / ng);
+  const int group = (int)(local - (long long)c.slot * ng);
+  c.tile = c.J->tile_of[c.slot];
+  const int lane = threadIdx.x & 63;
+  c.k = group * 64 + lane;
+  c.k_ok = c.k < c.J->n_axes;
+  if (!c.k_ok) c.k = c.J->n_axes - 1;
+  return true;
+}
+
+__device__ __forceinline__ void write_pose(const WaveCtx& c, double E) {
+  const Job& J = *c.J;
+  const int cid = J.center_id[c.slot];
+  if (c.k_ok) {
+    if (J.out64) J.out64[(long long)cid * J.n_axes + c.k] = E;
+    if (J.out32) J.out32[(long long)c.k * J.n_centers + cid] = (float)E;
+    if (!isfinite(E)) atomicOr(J.bad, 1);
+  }
+  if (J.marg32 && J.n_groups == 1) {
+    // marginal = -T (logsumexp(-E/T) - ln K)   (reference physics.py:126-130)
+    const double T = J.temperature;
+    const double a = c.k_ok ? -E / T : -INFINITY;
+    const double m = wave_max(a);
+    const double s = wave_sum(c.k_ok ? exp(a - m) : 0.0);
+    if ((threadIdx.x & 63) == 0) {
+      const double marg = -T * (m + log(s) - log((double)J.n_axes));
+      const float mf = (float)marg;
+      J.marg32[cid] = mf;
+      if (J.grid32) J.grid32[cid] = asinhf(mf / 1000.0f);
+      if (!isfinite(marg)) atomicOr(J.bad, 1);
+    }
+  }
+}
+
+// FP64 pair term with the reference's exact cutoff/floor semantics
+// (kernel.cpp:29-36).  d = o u - r where r = image - center.
+__device__ __forceinline__ double pair64(double o, double ux, double uy, double uz, double rx,
+                                         double ry, double rz, double rc2, double c12, double c6,
+                                         double sh, double fl2) {
+  const double dx = o * ux - rx, dy = o * uy - ry, dz = o * uz - rz;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  double e = 0.0;
+  if (d2 < rc2) {
+    const double safe = d2 < fl2 ? fl2 : d2;
+    const double inv = 1.0 / safe;
+    const double i6 = inv * inv * inv;
+    e = c12 * i6 * i6 - c6 * i6 + sh;
+  }
+  return e;
+}
+
+template <bool SYM>
+__global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict__ jobs, int n_jobs,
+                                                          const long long* __restrict__ job_work,
+                                                          long long n_work,
+                                                          const long long* __restrict__ offsets,
+                                                          const float4* __restrict__ list32,
+                                                          const double* __restrict__ list64) {
+  __shared__ float4 s_far[4][64];
+  __shared__ int s_near[4][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  WaveCtx c;
+  if (!setup_wave(jobs, n_jobs, job_work, n_work, wv, c)) return;
+  const Job& J = *c.J;
+
+  const double ux64 = J.axes64[3 * c.k], uy64 = J.axes64[3 * c.k + 1], uz64 = J.axes64[3 * c.k + 2];
+  const float ux = (float)ux64, uy = (float)uy64, uz = (float)uz64;
+  const double cx = J.center_rel[3 * c.slot], cy = J.center_rel[3 * c.slot + 1],
+               cz = J.center_rel[3 * c.slot + 2];
+  const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
+  const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
+  const float o0sq = J.o0sqf, dR2 = J.dR2f;
+  const double o0 = J.o0, o1 = J.o1, rc2 = J.rc2;
+  const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
+
+  double E = 0.0;
+  for (int t = 0; t < J.n_types; ++t) {
+    const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
+    const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
+    const double c6 = J.c6[t];
+    long long kept_far = 0;
+    for (long long base = beg; base < end; base += 64) {
+      const long long idx = base + lane;
+      const bool valid = idx < end;
+      float rx = 0.f, ry = 0.f, rz = 0.f, rr = 3.0e38f;
+      if (valid) {
+        const float4 e = list32[idx];
+        rx = e.x - cxf; ry = e.y - cyf; rz = e.z - czf;
+        rr = rx * rx + ry * ry + rz * rz;
+      }
+      const bool keep = rr < keep2f;
+      const bool is_near = keep && rr < near2;
+      const bool is_far = keep && !is_near;
+      const unsigned long long mfar = __ballot(is_far);
+      const unsigned long long mnear = __ballot(is_near);
+      if (is_far) s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, rr + o0sq);
+      if (is_near) s_near[wv][lanes_below(mnear)] = (int)(idx - base);
+      wave_sync();
+      const int nf = __popcll(mfar), nn = __popcll(mnear);
+      kept_far += nf;
+      float acc = 0.f;
+#pragma unroll 2
+      for (int q = 0; q < nf; ++q) {
+        const float4 f = s_far[wv][q];
+        const float tdot = f.x * ux + f.y * uy + f.z * uz;
+        float d0 = fmaf(tdot, m2o0, f.w);
+        float d1 = fmaf(tdot, m2o1, SYM ? f.w : f.w + dR2);
+        d0 = __builtin_amdgcn_fmed3f(d0, fl2f, rc2f);
+        d1 = __builtin_amdgcn_fmed3f(d1, fl2f, rc2f);
+        const float i0 = __builtin_amdgcn_rcpf(d0);
+        const float i1 = __builtin_amdgcn_rcpf(d1);
+        const float i06 = i0 * i0 * i0;
+        const float i16 = i1 * i1 * i1;
+        acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
+        acc = fmaf(i16, fmaf(i16, s6, -1.0f), acc);
+      }
+      E = fma(c6, (double)acc, E);
+      if (nn) {
+        const double c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
+        for (int q = 0; q < nn; ++q) {
+          const long long id = base + s_near[wv][q];
+          const double rx64 = list64[3 * id] - cx, ry64 = list64[3 * id + 1] - cy,
+                       rz64 = list64[3 * id + 2] - cz;
+          E += pair64(o0, ux64, uy64, uz64, rx64, ry64, rz64, rc2, c12, J.c6[t], sh, fl2);
+          E += pair64(o1, ux64, uy64, uz64, rx64, ry64, rz64, rc2, c12, J.c6[t], sh, fl2);
+        }
+      }
+      wave_sync();
+    }
+    // Every kept far pair was clamped to rc^2 at worst, contributing -shift.
+    E = fma((double)(2 * kept_far), J.shift[t], E);
+  }
+  write_pose(c, E);
+}
+
+__global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__ jobs, int n_jobs,
+                                                         const long long* __restrict__ job_work,
+                                                         long long n_work,
+                                                         const long long* __restrict__ offsets,
+                                                         const double* __restrict__ list64) {
+  __shared__ double s_r[4][3][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  WaveCtx c;
+  if (!setup_wave(jobs, n_jobs, job_work, n_work, wv, c)) return;
+  const Job& J = *c.J;
+  const double ux = J.axes64[3 * c.k], uy = J.axes64[3 * c.k + 1], uz = J.axes64[3 * c.k + 2];
+  const double cx = J.center_rel[3 * c.slot], cy = J.center_rel[3 * c.slot + 1],
+               cz = J.center_rel[3 * c.slot + 2];
+  const double keep2 = J.keep_r * J.keep_r, rc2 = J.rc2, o0 = J.o0, o1 = J.o1;
+  const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
+  double E = 0.0;
+  for (int t = 0; t < J.n_types; ++t) {
+    const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
+    const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
+    for (long long base = beg; base < end; base += 64) {
+      const long long idx = base + lane;
+      double rx = 0.0, ry = 0.0, rz = 0.0;
+      bool keep = false;
+      if (idx < end) {
+        rx = list64[3 * idx] - cx; ry = list64[3 * idx + 1] - cy; rz = list64[3 * idx + 2] - cz;
+        keep = rx * rx + ry * ry + rz * rz < keep2;
+      }
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const int p = lanes_below(m);
+        s_r[wv][0][p] = rx; s_r[wv][1][p] = ry; s_r[wv][2][p] = rz;
+      }
+      wave_sync();
+      const int n = __popcll(m);
+      for (int q = 0; q < n; ++q) {
+        const double ax = s_r[wv][0][q], ay = s_r[wv][1][q], az = s_r[wv][2][q];
+        E += pair64(o0, ux, uy, uz, ax, ay, az, rc2, c12, c6, sh, fl2);
+        E += pair64(o1, ux, uy, uz, ax, ay, az, rc2, c12, c6, sh, fl2);
+      }
+      wave_sync();
+    }
+  }
+  write_pose(c, E);
+}
+
+// Exact count of in-cutoff (site, image) pairs: the pair-eval census that the
+// throughput metric is quoted in.  Same traversal as field_fp64_kernel.
+__global__ __launch_bounds__(256) void census_kernel(const Job* __restrict__ jobs, int n_jobs,
+                                                     const long long* __restrict__ job_work,
+                                                     long long n_work,
+                                                     const long long* __restrict__ offsets,
+                                                     const double* __restrict__ list64) {
+  __shared__ double s_r[4][3][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  WaveCtx c;
+  if (!setup_wave(jobs, n_jobs, job_work, n_work, wv, c)) return;
+  const Job& J = *c.J;
+  const double ux = J.axes64[3 * c.k], uy = J.axes64[3 * c.k + 1], uz = J.axes64[3 * c.k + 2];
+  const double cx = J.center_rel[3 * c.slot], cy = J.center_rel[3 * c.slot + 1],
+               cz = J.center_rel[3 * c.slot + 2];
+  const double keep2 = J.keep_r * J.keep_r, rc2 = J.rc2, o0 = J.o0, o1 = J.o1;
+  const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
+  const long long beg = offsets[seg0], end = offsets[seg0 + J.n_types];
+  unsigned long long pairs = 0;
+  for (long long base = beg; base < end; base += 64) {
+    const long long idx = base + lane;
+    double rx = 0.0, ry = 0.0, rz = 0.0;
+    bool keep = false;
+    if (idx < end) {
+      rx = list64[3 * idx] - cx; ry = list64[3 * idx + 1] - cy; rz = list64[3 * idx + 2] - cz;
+      keep = rx * rx + ry * ry + rz * rz < keep2;
+    }
+    const unsigned long long m = __ballot(keep);
+    if (keep) {
+      const int p = lanes_below(m);
+      s_r[wv][0][p] = rx; s_r[wv][1][p] = ry; s_r[wv][2][p] = rz;
+    }
+    wave_sync();
+    const int n = __popcll(m);
+    for (int q = 0; q < n; ++q) {
+      const double ax = s_r[wv][0][q], ay = s_r[wv][1][q], az = s_r[wv][2][q];
+      const double a0 = o0 * ux - ax, b0 = o0 * uy - ay, g0 = o0 * uz - az;
+      const double a1 = o1 * ux - ax, b1 = o1 * uy - ay, g1 = o1 * uz - az;
+      pairs += (a0 * a0 + b0 * b0 + g0 * g0 < rc2);
+      pairs += (a1 * a1 + b1 * b1 + g1 * g1 < rc2);
+    }
+    wave_sync();
+  }
+  if (!c.k_ok) pairs = 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o, 64);
+  if (lane == 0) atomicAdd(J.pair_count, pairs);
+}
+
+// Marginal over orientations for jobs with more than 64 axes (the field kernel
+// fuses it when one wave holds all of a center's orientations).
+__global__ __launch_bounds__(64) void marginal_kernel(const Job* __restrict__ jobs, int n_jobs,
+                                                      const long long* __restrict__ job_first,
+                                                      long long n_total) {
+  const long long g = blockIdx.x;
+  if (g >= n_total) return;
+  const int j = find_job(job_first, n_jobs, g);
+  const Job& J = jobs[j];
+  const int cid = (int)(g - job_first[j]);
+  const int lane = threadIdx.x;
+  const double T = J.temperature;
+  double m = -INFINITY;
+  for (int k = lane; k < J.n_axes; k += 64) m = fmax(m, -J.out64[(long long)cid * J.n_axes + k] / T);
+  m = wave_max(m);
+  double s = 0.0;
+  for (int k = lane; k < J.n_axes; k += 64) s += exp(-J.out64[(long long)cid * J.n_axes + k] / T - m);
+  s = wave_sum(s);
+  if (lane == 0) {
+    const double marg = -T * (m + log(s) - log((double)J.n_axes));
+    const float mf = (float)marg;
+    J.marg32[cid] = mf;
+    if (J.grid32) J.grid32[cid] = asinhf(mf / 1000.0f);
+    if (!isfinite(marg)) atomicOr(J.bad, 1);
+  }
+}
+
+// Single-site field at every center (reference physics.py:59-85 with the
+// case cutoff): r = |d|, ratio = (sigma / max(r, 0.1 sigma))^6,
+// 4 eps (ratio^2 - ratio) + shift for r < rc.  Lanes are member centers of
+// one tile; the tile list is read by broadcast.
+__global__ __launch_bounds__(64) void site_kernel(const Job* __restrict__ jobs,
+                                                  const SiteChunk* __restrict__ chunks,
+                                                  const long long* __restrict__ offsets,
+                                                  const double* __restrict__ list64) {
+  const SiteChunk ch = chunks[blockIdx.x];
+  const Job& J = jobs[ch.job];
+  const int lane = threadIdx.x;
+  const bool ok = lane < ch.count;
+  const int slot = ch.first + (ok ? lane : 0);
+  const double cx = J.center_rel[3 * slot], cy = J.center_rel[3 * slot + 1],
+               cz = J.center_rel[3 * slot + 2];
+  const long long seg0 = J.seg_base + (long long)ch.tile * J.n_types;
+  const double rc2 = J.rc2;
+  double E = 0.0;
+  for (int t = 0; t < J.n_types; ++t) {
+    const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
+    const double sg = J.sigma[t], e4 = 4.0 * J.eps[t], sh = J.shift[t];
+    for (long long i = beg; i < end; ++i) {
+      const double dx = list64[3 * i] - cx, dy = list64[3 * i + 1] - cy,
+                   dz = list64[3 * i + 2] - cz;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < rc2) {
+        const double r = sqrt(d2);
+        const double q = sg / fmax(r, 0.1 * sg);
+        const double q2 = q * q;
+        const double ratio = q2 * q2 * q2;
+        E += e4 * (ratio * ratio - ratio) + sh;
+      }
+    }
+  }
+  if (ok) {
+    J.site32[J.center_id[slot]] = (float)E;
+    if (!isfinite(E)) atomicOr(J.bad, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in CSR kernel: evaluate_reused_neighbors (reference kernel.cpp:5-45).
+// Same operation order, IEEE division and no FMA contraction, so a center's
+// energies reproduce the reference's x86-64 results bit for bit on the same
+// CSR.  One wave per (center, group of 64 axes).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void legacy_kernel(int n_centers, int n_axes, int n_groups,
+                                                     double cutoff2,
+                                                     const double* __restrict__ centers,
+                                                     const double* __restrict__ axes,
+                                                     const double* __restrict__ images,
+                                                     const double* __restrict__ offsets,
+                                                     const double* __restrict__ coeff12,
+                                                     const double* __restrict__ coeff6,
+                                                     const double* __restrict__ shift,
+                                                     const double* __restrict__ floor2,
+                                                     const long long* __restrict__ starts,
+                                                     const int* __restrict__ indices,
+                                                     double* __restrict__ output,
+                                                     int* __restrict__ bad_center) {
+#pragma clang fp contract(off)
+  const int wv = uniform(threadIdx.x >> 6);
+  const long long w = (long long)blockIdx.x * 4 + wv;
+  if (w >= (long long)n_centers * n_groups) return;
+  const int center = (int)(w / n_groups);
+  const int k = (int)(w % n_groups) * 64 + (threadIdx.x & 63);
+  const bool ok = k < n_axes;
+  const int kk = ok ? k : n_axes - 1;
+  const long long n0 = starts[center], n1 = starts[center + 1];
+  double energy = 0.0;
+  for (int site = 0; site < 2; ++site) {
+    const double x = centers[3 * center] + offsets[site] * axes[3 * kk];
+    const double y = centers[3 * center + 1] + offsets[site] * axes[3 * kk + 1];
+    const double z = centers[3 * center + 2] + offsets[site] * axes[3 * kk + 2];
+    for (long long n = n0; n < n1; ++n) {
+      const int j = indices[n];
+      const double dx = x - images[3 * j];
+      const double dy = y - images[3 * j + 1];
+      const double dz = z - images[3 * j + 2];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 >= cutoff2) continue;
+      const double safe = d2 < floor2[j] ? floor2[j] : d2;
+      const double i2 = 1.0 / safe;
+      const double i6 = i2 * i2 * i2;
+      energy += coeff12[j] * i6 * i6 - coeff6[j] * i6 + shift[j];
+    }
+  }
+  if (ok) {
+    output[(long long)center * n_axes + k] = energy;
+    if (!isfinite(energy)) atomicMin(bad_center, center);
+  }
+}
+
+__global__ void zero_kernel(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+// Clearing flags with a kernel rather than hipMemsetAsync keeps every step of
+// a run capturable in a HIP graph.
+hipError_t launch_zero(int* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_count(const Job* d_jobs, const Tile* d_tiles, int n_tiles_total,
+                             const long long* d_seg_base, int* d_counts, hipStream_t s) {
+  if (n_tiles_total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(tile_count_kernel, dim3(n_tiles_total), dim3(64), 0, s, d_jobs, d_tiles,
+                     d_seg_base, d_counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const int* d_counts, long long n, long long* d_offsets, hipStream_t s) {
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, d_counts, n, d_offsets);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_fill(const Job* d_jobs, const Tile* d_tiles, int n_tiles_total,
+                            const long long* d_seg_base, const long long* d_offsets,
+                            float4* d_list32, double* d_list64, hipStream_t s) {
+  if (n_tiles_total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(tile_fill_kernel, dim3(n_tiles_total), dim3(64), 0, s, d_jobs, d_tiles,
+                     d_seg_base, d_offsets, d_list32, d_list64);
+  return hipGetLastError();
+}
+
+hipError_t launch_field(int precision, const Job* d_jobs, int n_jobs, const long long* d_job_work,
+                        long long n_work, const long long* d_offsets, const float4* d_list32,
+                        const double* d_list64, hipStream_t s, bool all_sym) {
+  if (n_work <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((n_work + 3) / 4)), block(256);
+  if (precision == PREC_MIXED) {
+    if (all_sym)
+      hipLaunchKernelGGL(field_mixed_kernel<true>, grid, block, 0, s, d_jobs, n_jobs, d_job_work,
+                         n_work, d_offsets, d_list32, d_list64);
+    else
+      hipLaunchKernelGGL(field_mixed_kernel<false>, grid, block, 0, s, d_jobs, n_jobs, d_job_work,
+                         n_work, d_offsets, d_list32, d_list64);
+  } else {
+    hipLaunchKernelGGL(field_fp64_kernel, grid, block, 0, s, d_jobs, n_jobs, d_job_work, n_work,
+                       d_offsets, d_list64);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_census(const Job* d_jobs, int n_jobs, const long long* d_job_work,
+                         long long n_work, const long long* d_offsets, const double* d_list64,
+                         hipStream_t s) {
+  if (n_work <= 0) return hipSuccess;
+  hipLaunchKernelGGL(census_kernel, dim3((unsigned)((n_work + 3) / 4)), dim3(256), 0, s, d_jobs,
+                     n_jobs, d_job_work, n_work, d_offsets, d_list64);
+  return hipGetLastError();
+}
+
+hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job_first,
+                           long long n_total, hipStream_t s) {
+  if (n_total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(marginal_kernel, dim3((unsigned)n_total), dim3(64), 0, s, d_jobs, n_jobs,
+                     d_job_first, n_total);
+  return hipGetLastError();
+}
+
+hipError_t launch_site(const Job* d_jobs, const SiteChunk* d_chunks, int n_chunks,
+                       const long long* d_offsets, const double* d_list64, hipStream_t s) {
+  if (n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(site_kernel, dim3(n_chunks), dim3(64), 0, s, d_jobs, d_chunks, d_offsets,
+                     d_list64);
+  return hipGetLastError();
+}
+
+hipError_t launch_legacy(int n_centers, int n_axes, double cutoff2, const double* centers,
+                         const double* axes, const double* images, const double* offsets,
+                         const double* coeff12, const double* coeff6, const double* shift,
+                         const double* floor2, const long long* starts, const int* indices,
+                         double* output, int* bad_center, hipStream_t s) {
+  const int n_groups = (n_axes + 63) / 64;
+  const long long waves = (long long)n_centers * n_groups;
+  if (waves <= 0) return hipSuccess;
+  hipLaunchKernelGGL(legacy_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, n_centers,
+                     n_axes, n_groups, cutoff2, centers, axes, images, offsets, coeff12, coeff6,
+                     shift, floor2, starts, indices, output, bad_center);
+  return hipGetLastError();
+}
+
+}  // namespace sv

@@ -1,0 +1,888 @@
+// Host runtime of libsv_vext.so: FP64 geometry setup, pose tiling, device
+// tables, launch orchestration and the C ABI declared in include/sv_vext.h.
+#include "sv_internal.h"
+#include "sv_vext.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace sv {
+
+hipError_t launch_zero(int* p, int n, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
+
+namespace {
+
+thread_local std::string t_error;
+
+int fail(int code, const std::string& msg) {
+  t_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? 6 : 5, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define SV_HIP(call)                                   \
+  do {                                                 \
+    hipError_t sv_e_ = (call);                         \
+    if (sv_e_ != hipSuccess) return hip_fail(sv_e_, #call); \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  hipError_t reserve(size_t n, bool grow = false) {
+    if (n <= cap && p) return hipSuccess;
+    release();
+    size_t want = std::max<size_t>(n, 1);
+    if (grow) want += want / 4;
+    hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    else p = nullptr;
+    return e;
+  }
+  hipError_t upload(const T* h, size_t n, hipStream_t s) {
+    hipError_t e = reserve(n);
+    if (e != hipSuccess || n == 0) return e;
+    return hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+// numpy's np.mod(x, 1.0) for finite x (result in [0, 1], 1.0 only for tiny
+// negative x), used by the reference to wrap fractional coordinates.
+double wrap01(double x) {
+  double r = std::fmod(x, 1.0);
+  if (r != 0.0 && r < 0.0) r += 1.0;
+  return r;
+}
+
+bool invert3(const double* H, double* inv, double* det_out) {
+  const double a = H[0], b = H[1], c = H[2], d = H[3], e = H[4], f = H[5], g = H[6], h = H[7],
+               i = H[8];
+  const double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+  const double det = a * A + b * B + c * C;
+  *det_out = det;
+  if (!std::isfinite(det) || det == 0.0) return false;
+  inv[0] = A / det;
+  inv[3] = B / det;
+  inv[6] = C / det;
+  inv[1] = -(b * i - c * h) / det;
+  inv[4] = (a * i - c * g) / det;
+  inv[7] = -(a * h - b * g) / det;
+  inv[2] = (b * f - c * e) / det;
+  inv[5] = -(a * f - c * d) / det;
+  inv[8] = (a * e - b * d) / det;
+  return true;
+}
+
+}  // namespace
+
+struct Plan {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double H[9], Hinv[9], hn[3], volume;
+  int n_atoms = 0, n_types = 0;
+  std::vector<double> frac;       // sorted by type, wrapped
+  std::vector<int> type_start;
+  std::vector<double> c12, c6, shift, floor2, sigma, eps;
+  std::vector<float> s6f, floor2f, near2f;
+  double o0, o1, cutoff, keep_r, near_energy;
+  DevBuf<double> d_frac, d_c12, d_c6, d_shift, d_floor2, d_sigma, d_eps;
+  DevBuf<int> d_type_start;
+  DevBuf<float> d_s6f, d_floor2f, d_near2f;
+  ~Plan() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct Poses {
+  Plan* plan = nullptr;
+  int n_centers = 0, n_axes = 0, n_groups = 0;
+  std::vector<double> axes;
+  std::vector<Tile> tiles;        // job field left 0; filled per batch
+  std::vector<int> center_id, tile_of;
+  std::vector<double> center_rel;
+  DevBuf<double> d_center_rel, d_axes64;
+  DevBuf<float> d_axes32;
+  DevBuf<int> d_center_id, d_tile_of;
+  DevBuf<double> w_out64;
+  DevBuf<float> w_out32, w_marg, w_grid, w_site;
+};
+
+struct Batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_jobs = 0, precision = 0;
+  bool all_sym = true, need_marginal = false, any_site = false;
+  std::vector<Job> jobs;
+  std::vector<long long> job_work, job_first;
+  std::vector<Tile> tiles;
+  std::vector<long long> seg_base;
+  std::vector<SiteChunk> chunks;
+  long long n_segs = 0, n_work = 0, n_centers_total = 0, list_total = -1;
+  DevBuf<Job> d_jobs;
+  DevBuf<long long> d_job_work, d_job_first, d_seg_base, d_offsets;
+  DevBuf<Tile> d_tiles;
+  DevBuf<SiteChunk> d_chunks;
+  DevBuf<int> d_counts, d_bad;
+  DevBuf<float4> d_list32;
+  DevBuf<double> d_list64;
+  DevBuf<unsigned long long> d_pairs;
+  hipEvent_t ev[12] = {};
+  bool timed_last = false;
+  ~Batch() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Tile counts per lattice direction: tiles of roughly 64 centers, about as
+// thick along every direction.  A regular grid of `grid` points per axis gets
+// a divisor of `grid` so that tiles hold whole blocks of grid points.
+void choose_tiles(const Plan& P, long long n_centers, int grid, int m[3]) {
+  const double rho = (double)n_centers / P.volume;
+  const double edge = std::cbrt(64.0 / std::max(rho, 1e-30));
+  for (int k = 0; k < 3; ++k) {
+    const double thick = 1.0 / P.hn[k];
+    double want = thick / edge;
+    if (want < 1.0) want = 1.0;
+    if (grid > 0) {
+      int best = 1;
+      double best_err = 1e300;
+      for (int d = 1; d <= grid; ++d) {
+        if (grid % d) continue;
+        const double err = std::fabs(std::log((double)d / want));
+        if (err < best_err) best_err = err, best = d;
+      }
+      m[k] = best;
+    } else {
+      m[k] = (int)std::min(1024.0, std::max(1.0, std::floor(want + 0.5)));
+    }
+  }
+}
+
+int build_poses(Plan* P, int n, const double* frac_in, int n_axes, const double* axes, int grid,
+                Poses** out) {
+  if (n < 0 || n_axes < 1) return fail(1, "n_centers >= 0 and n_axes >= 1 required");
+  std::vector<double> f(3 * (size_t)n);
+  for (size_t i = 0; i < 3 * (size_t)n; ++i) {
+    if (!std::isfinite(frac_in[i])) return fail(1, "nonfinite center");
+    f[i] = wrap01(frac_in[i]);
+  }
+  auto S = new Poses();
+  S->plan = P;
+  S->n_centers = n;
+  S->n_axes = n_axes;
+  S->n_groups = (n_axes + 63) / 64;
+  S->axes.assign(axes, axes + 3 * (size_t)n_axes);
+  for (int k = 0; k < n_axes; ++k) {
+    const double* u = axes + 3 * k;
+    const double nn = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    if (!std::isfinite(nn) || nn <= 0.0) {
+      delete S;
+      return fail(1, "finite nonzero directors required");
+    }
+  }
+  int m[3];
+  choose_tiles(*P, std::max(n, 1), grid, m);
+  const long long nbins = (long long)m[0] * m[1] * m[2];
+  std::vector<int> bin(n);
+  std::vector<long long> cnt(nbins + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    int idx[3];
+    for (int k = 0; k < 3; ++k) idx[k] = std::min((int)(f[3 * i + k] * m[k]), m[k] - 1);
+    bin[i] = (idx[0] * m[1] + idx[1]) * m[2] + idx[2];
+    ++cnt[bin[i] + 1];
+  }
+  for (long long b = 0; b < nbins; ++b) cnt[b + 1] += cnt[b];
+  std::vector<int> order(n);
+  {
+    std::vector<long long> pos(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < n; ++i) order[pos[bin[i]]++] = i;
+  }
+  std::vector<int> tile_of_bin(nbins, -1);
+  for (long long b = 0; b < nbins; ++b) {
+    if (cnt[b + 1] == cnt[b]) continue;
+    Tile T{};
+    const int ia = (int)(b / ((long long)m[1] * m[2]));
+    const int ib = (int)((b / m[2]) % m[1]);
+    const int ic = (int)(b % m[2]);
+    T.frac[0] = (ia + 0.5) / m[0];
+    T.frac[1] = (ib + 0.5) / m[1];
+    T.frac[2] = (ic + 0.5) / m[2];
+    T.first_member = (int)cnt[b];
+    T.n_members = (int)(cnt[b + 1] - cnt[b]);
+    tile_of_bin[b] = (int)S->tiles.size();
+    S->tiles.push_back(T);
+  }
+  S->center_id = order;
+  S->tile_of.resize(n);
+  S->center_rel.resize(3 * (size_t)n);
+  for (int s = 0; s < n; ++s) {
+    const int i = order[s];
+    const int t = tile_of_bin[bin[i]];
+    S->tile_of[s] = t;
+    Tile& T = S->tiles[t];
+    const double da = f[3 * i] - T.frac[0], db = f[3 * i + 1] - T.frac[1],
+                 dc = f[3 * i + 2] - T.frac[2];
+    double r2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double x = da * P->H[k] + db * P->H[3 + k] + dc * P->H[6 + k];
+      S->center_rel[3 * (size_t)s + k] = x;
+      r2 += x * x;
+    }
+    T.radius = std::max(T.radius, std::sqrt(r2));
+  }
+  for (Tile& T : S->tiles) T.list_r = T.radius + P->keep_r + 1e-3;
+
+  hipError_t e;
+  std::vector<float> a32(3 * (size_t)n_axes);
+  for (size_t i = 0; i < a32.size(); ++i) a32[i] = (float)S->axes[i];
+  if ((e = hipSetDevice(P->device)) != hipSuccess ||
+      (e = S->d_center_rel.upload(S->center_rel.data(), S->center_rel.size(), P->stream)) != hipSuccess ||
+      (e = S->d_center_id.upload(S->center_id.data(), n, P->stream)) != hipSuccess ||
+      (e = S->d_tile_of.upload(S->tile_of.data(), n, P->stream)) != hipSuccess ||
+      (e = S->d_axes64.upload(S->axes.data(), S->axes.size(), P->stream)) != hipSuccess ||
+      (e = S->d_axes32.upload(a32.data(), a32.size(), P->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
+    delete S;
+    return hip_fail(e, "pose upload");
+  }
+  *out = S;
+  return 0;
+}
+
+int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precision,
+                const double* temps, double* const* energy, float* const* orientation,
+                float* const* marginal, float* const* grid, float* const* site, Batch** out) {
+  if (n_jobs < 1) return fail(1, "at least one job required");
+  if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
+  const int device = plans[0]->device;
+  auto B = new Batch();
+  B->device = device;
+  B->n_jobs = n_jobs;
+  B->precision = precision;
+  B->jobs.resize(n_jobs);
+  B->job_work.resize(n_jobs + 1, 0);
+  B->job_first.resize(n_jobs + 1, 0);
+  long long segs = 0, work = 0, centers = 0;
+  for (int j = 0; j < n_jobs; ++j) {
+    Plan* P = plans[j];
+    Poses* S = poses[j];
+    if (!P || !S || S->plan != P || P->device != device) {
+      delete B;
+      return fail(1, "each job needs a plan and a pose set created from it, on one device");
+    }
+    Job& J = B->jobs[j];
+    std::memset(&J, 0, sizeof(Job));
+    std::memcpy(J.H, P->H, sizeof(J.H));
+    std::memcpy(J.hn, P->hn, sizeof(J.hn));
+    J.frac = P->d_frac.p;
+    J.type_start = P->d_type_start.p;
+    J.n_atoms = P->n_atoms;
+    J.n_types = P->n_types;
+    J.o0 = P->o0;
+    J.o1 = P->o1;
+    J.rc2 = P->cutoff * P->cutoff;
+    J.keep_r = P->keep_r;
+    J.rc2f = (float)J.rc2;
+    J.keep2f = (float)(P->keep_r * P->keep_r);
+    J.m2o0f = (float)(-2.0 * P->o0);
+    J.m2o1f = (float)(-2.0 * P->o1);
+    J.o0sqf = (float)(P->o0 * P->o0);
+    J.dR2f = (float)(P->o1 * P->o1 - P->o0 * P->o0);
+    J.sym = (P->o1 == -P->o0);
+    if (!J.sym) B->all_sym = false;
+    J.c12 = P->d_c12.p;
+    J.c6 = P->d_c6.p;
+    J.shift = P->d_shift.p;
+    J.floor2 = P->d_floor2.p;
+    J.sigma = P->d_sigma.p;
+    J.eps = P->d_eps.p;
+    J.s6f = P->d_s6f.p;
+    J.floor2f = P->d_floor2f.p;
+    J.near2f = P->d_near2f.p;
+    J.center_rel = S->d_center_rel.p;
+    J.center_id = S->d_center_id.p;
+    J.tile_of = S->d_tile_of.p;
+    J.axes64 = S->d_axes64.p;
+    J.axes32 = S->d_axes32.p;
+    J.n_centers = S->n_centers;
+    J.n_axes = S->n_axes;
+    J.n_groups = S->n_groups;
+    J.n_tiles = (int)S->tiles.size();
+    J.tile_base = (int)B->tiles.size();
+    J.seg_base = segs;
+    J.work_begin = work;
+    J.out64 = energy ? energy[j] : nullptr;
+    J.out32 = orientation ? orientation[j] : nullptr;
+    J.marg32 = marginal ? marginal[j] : nullptr;
+    J.grid32 = grid ? grid[j] : nullptr;
+    J.site32 = site ? site[j] : nullptr;
+    J.temperature = temps ? temps[j] : 0.0;
+    if ((J.marg32 || J.grid32) && !(temps && std::isfinite(temps[j]) && temps[j] > 0.0)) {
+      delete B;
+      return fail(1, "positive finite temperature required for the marginal");
+    }
+    if (J.grid32 && !J.marg32) {
+      delete B;
+      return fail(1, "grid output requires the marginal output");
+    }
+    if (J.marg32 && S->n_groups > 1) {
+      if (!J.out64) {
+        delete B;
+        return fail(1, "marginal over more than 64 axes requires the FP64 energy output");
+      }
+      B->need_marginal = true;
+    }
+    if (J.site32) B->any_site = true;
+    for (size_t t = 0; t < S->tiles.size(); ++t) {
+      Tile T = S->tiles[t];
+      T.job = j;
+      B->tiles.push_back(T);
+      B->seg_base.push_back(segs + (long long)t * P->n_types);
+      if (J.site32)
+        for (int first = 0; first < T.n_members; first += 64)
+          B->chunks.push_back({j, (int)t, T.first_member + first, std::min(64, T.n_members - first)});
+    }
+    segs += (long long)S->tiles.size() * P->n_types;
+    work += (long long)S->n_centers * S->n_groups;
+    B->job_first[j] = centers;
+    centers += S->n_centers;
+    B->job_work[j + 1] = work;
+    B->job_first[j + 1] = centers;
+  }
+  B->n_segs = segs;
+  B->n_work = work;
+  B->n_centers_total = centers;
+  hipError_t e;
+  Batch* b = B;
+  if ((e = hipSetDevice(device)) != hipSuccess ||
+      (e = hipStreamCreate(&b->stream)) != hipSuccess ||
+      (e = b->d_jobs.upload(b->jobs.data(), b->jobs.size(), b->stream)) != hipSuccess ||
+      (e = b->d_job_work.upload(b->job_work.data(), b->job_work.size(), b->stream)) != hipSuccess ||
+      (e = b->d_job_first.upload(b->job_first.data(), b->job_first.size(), b->stream)) != hipSuccess ||
+      (e = b->d_tiles.upload(b->tiles.data(), b->tiles.size(), b->stream)) != hipSuccess ||
+      (e = b->d_seg_base.upload(b->seg_base.data(), b->seg_base.size(), b->stream)) != hipSuccess ||
+      (e = b->d_chunks.upload(b->chunks.data(), b->chunks.size(), b->stream)) != hipSuccess ||
+      (e = b->d_counts.reserve(std::max<long long>(segs, 1))) != hipSuccess ||
+      (e = b->d_offsets.reserve(segs + 1)) != hipSuccess ||
+      (e = b->d_bad.reserve(1)) != hipSuccess ||
+      (e = b->d_pairs.reserve(n_jobs)) != hipSuccess) {
+    delete B;
+    return hip_fail(e, "batch setup");
+  }
+  for (int j = 0; j < n_jobs; ++j) {
+    b->jobs[j].bad = b->d_bad.p;
+    b->jobs[j].pair_count = b->d_pairs.p + j;
+  }
+  if ((e = b->d_jobs.upload(b->jobs.data(), b->jobs.size(), b->stream)) != hipSuccess) {
+    delete B;
+    return hip_fail(e, "batch job table");
+  }
+  for (auto& ev : b->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) {
+      delete B;
+      return hip_fail(e, "hipEventCreate");
+    }
+  if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) {
+    delete B;
+    return hip_fail(e, "batch upload");
+  }
+  *out = B;
+  return 0;
+}
+
+// Candidate lists: count -> scan -> fill.  The list size of a batch never
+// changes, so only the first run reads it back (one host sync); later runs are
+// fully asynchronous.
+int batch_lists(Batch* B, bool timed) {
+  const int nt = (int)B->tiles.size();
+  if (timed) SV_HIP(hipEventRecord(B->ev[0], B->stream));
+  SV_HIP(launch_tile_count(B->d_jobs.p, B->d_tiles.p, nt, B->d_seg_base.p, B->d_counts.p, B->stream));
+  if (timed) SV_HIP(hipEventRecord(B->ev[1], B->stream));
+  SV_HIP(launch_scan(B->d_counts.p, B->n_segs, B->d_offsets.p, B->stream));
+  if (timed) SV_HIP(hipEventRecord(B->ev[2], B->stream));
+  if (B->list_total < 0) {
+    long long total = 0;
+    SV_HIP(hipMemcpyAsync(&total, B->d_offsets.p + B->n_segs, sizeof(total), hipMemcpyDeviceToHost,
+                          B->stream));
+    SV_HIP(hipStreamSynchronize(B->stream));
+    if (total < 0) return fail(5, "negative candidate list size");
+    SV_HIP(B->d_list32.reserve((size_t)std::max<long long>(total, 1)));
+    SV_HIP(B->d_list64.reserve(3 * (size_t)std::max<long long>(total, 1)));
+    B->list_total = total;
+  }
+  if (timed) SV_HIP(hipEventRecord(B->ev[3], B->stream));
+  SV_HIP(launch_tile_fill(B->d_jobs.p, B->d_tiles.p, nt, B->d_seg_base.p, B->d_offsets.p,
+                          B->d_list32.p, B->d_list64.p, B->stream));
+  if (timed) SV_HIP(hipEventRecord(B->ev[4], B->stream));
+  return 0;
+}
+
+int batch_run(Batch* B, bool timed, bool sync) {
+  SV_HIP(hipSetDevice(B->device));
+  int rc = batch_lists(B, timed);
+  if (rc) return rc;
+  SV_HIP(launch_zero(B->d_bad.p, 1, B->stream));
+  if (timed) SV_HIP(hipEventRecord(B->ev[5], B->stream));
+  SV_HIP(launch_field(B->precision, B->d_jobs.p, B->n_jobs, B->d_job_work.p, B->n_work,
+                      B->d_offsets.p, B->d_list32.p, B->d_list64.p, B->stream, B->all_sym));
+  if (timed) SV_HIP(hipEventRecord(B->ev[6], B->stream));
+  if (B->need_marginal)
+    SV_HIP(launch_marginal(B->d_jobs.p, B->n_jobs, B->d_job_first.p, B->n_centers_total, B->stream));
+  if (timed) SV_HIP(hipEventRecord(B->ev[7], B->stream));
+  if (B->any_site)
+    SV_HIP(launch_site(B->d_jobs.p, B->d_chunks.p, (int)B->chunks.size(), B->d_offsets.p,
+                       B->d_list64.p, B->stream));
+  if (timed) SV_HIP(hipEventRecord(B->ev[8], B->stream));
+  B->timed_last = timed;
+  if (!sync) return 0;
+  int bad = 0;
+  SV_HIP(hipMemcpyAsync(&bad, B->d_bad.p, sizeof(int), hipMemcpyDeviceToHost, B->stream));
+  SV_HIP(hipStreamSynchronize(B->stream));
+  if (bad) return fail(4, "nonfinite Vext energy");
+  return 0;
+}
+
+int plan_create(const double* cell, int n_atoms, const double* frac, const double* sigma,
+                const double* epsilon, double gs, double ge, const double* offsets, double cutoff,
+                double near_energy, Plan** out) {
+  if (!cell || !frac || !sigma || !epsilon || !offsets || n_atoms < 1)
+    return fail(1, "cell, atoms (n_atoms >= 1), LJ parameters and offsets required");
+  for (int i = 0; i < 9; ++i)
+    if (!std::isfinite(cell[i])) return fail(1, "nonfinite cell");
+  if (!(std::isfinite(cutoff) && cutoff > 0.0)) return fail(1, "positive finite cutoff required");
+  if (!(std::isfinite(gs) && gs > 0.0 && std::isfinite(ge) && ge > 0.0))
+    return fail(1, "positive guest sigma/epsilon required");
+  if (!(std::isfinite(offsets[0]) && std::isfinite(offsets[1])))
+    return fail(1, "finite site offsets required");
+  auto P = new Plan();
+  std::memcpy(P->H, cell, sizeof(P->H));
+  double det;
+  if (!invert3(P->H, P->Hinv, &det) || det <= 0.0) {
+    delete P;
+    return fail(1, "cell matrix must have a positive determinant");
+  }
+  P->volume = det;
+  for (int k = 0; k < 3; ++k)
+    P->hn[k] = std::sqrt(P->Hinv[k] * P->Hinv[k] + P->Hinv[3 + k] * P->Hinv[3 + k] +
+                         P->Hinv[6 + k] * P->Hinv[6 + k]);
+  P->n_atoms = n_atoms;
+  P->o0 = offsets[0];
+  P->o1 = offsets[1];
+  P->cutoff = cutoff;
+  P->keep_r = cutoff + std::max(std::fabs(P->o0), std::fabs(P->o1)) + 1e-3;
+  P->near_energy = near_energy;
+  // Distinct (sigma, epsilon) pairs in order of first appearance.
+  std::map<std::pair<double, double>, int> type_of;
+  std::vector<std::pair<double, double>> types;
+  std::vector<int> atype(n_atoms);
+  for (int i = 0; i < n_atoms; ++i) {
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(frac[3 * i + k])) {
+        delete P;
+        return fail(1, "nonfinite framework coordinate");
+      }
+    if (!(std::isfinite(sigma[i]) && sigma[i] > 0.0 && std::isfinite(epsilon[i]) && epsilon[i] > 0.0)) {
+      delete P;
+      return fail(1, "invalid framework LJ parameters");
+    }
+    auto key = std::make_pair(sigma[i], epsilon[i]);
+    auto it = type_of.find(key);
+    if (it == type_of.end()) {
+      it = type_of.emplace(key, (int)types.size()).first;
+      types.push_back(key);
+    }
+    atype[i] = it->second;
+  }
+  P->n_types = (int)types.size();
+  P->type_start.assign(P->n_types + 1, 0);
+  for (int i = 0; i < n_atoms; ++i) ++P->type_start[atype[i] + 1];
+  for (int t = 0; t < P->n_types; ++t) P->type_start[t + 1] += P->type_start[t];
+  P->frac.resize(3 * (size_t)n_atoms);
+  {
+    std::vector<int> pos(P->type_start.begin(), P->type_start.end() - 1);
+    for (int i = 0; i < n_atoms; ++i) {
+      const int s = pos[atype[i]]++;
+      for (int k = 0; k < 3; ++k) P->frac[3 * (size_t)s + k] = wrap01(frac[3 * i + k]);
+    }
+  }
+  // Mixed coefficients, same operation order as the reference numpy code
+  // (neighbor.py:21-28; physics.py:76-78).
+  const double omax = std::max(std::fabs(P->o0), std::fabs(P->o1));
+  for (int t = 0; t < P->n_types; ++t) {
+    const double ms = (types[t].first + gs) / 2.0;
+    const double me = std::sqrt(types[t].second * ge);
+    const double s6 = std::pow(ms, 6.0);
+    const double c12 = 4.0 * me * (s6 * s6);
+    const double c6 = 4.0 * me * s6;
+    const double cut6 = std::pow(ms / cutoff, 6.0);
+    const double sh = -4.0 * me * (cut6 * cut6 - cut6);
+    const double fl = (0.1 * ms) * (0.1 * ms);
+    P->c12.push_back(c12);
+    P->c6.push_back(c6);
+    P->shift.push_back(sh);
+    P->floor2.push_back(fl);
+    P->sigma.push_back(ms);
+    P->eps.push_back(me);
+    P->s6f.push_back((float)(c12 / c6));
+    P->floor2f.push_back((float)fl);
+    float near2 = 0.0f;
+    if (near_energy > 0.0) {
+      // Repulsive-branch distance where 4 eps (x^2 - x) = near_energy, x = (sigma/r)^6.
+      const double x = 0.5 * (1.0 + std::sqrt(1.0 + near_energy / me));
+      const double rstar = ms * std::pow(x, -1.0 / 6.0);
+      const double rn = rstar + omax;
+      near2 = (float)(rn * rn);
+    }
+    P->near2f.push_back(near2);
+  }
+  hipError_t e;
+  if ((e = hipGetDevice(&P->device)) != hipSuccess || (e = hipStreamCreate(&P->stream)) != hipSuccess ||
+      (e = P->d_frac.upload(P->frac.data(), P->frac.size(), P->stream)) != hipSuccess ||
+      (e = P->d_type_start.upload(P->type_start.data(), P->type_start.size(), P->stream)) != hipSuccess ||
+      (e = P->d_c12.upload(P->c12.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_c6.upload(P->c6.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_shift.upload(P->shift.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_floor2.upload(P->floor2.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_sigma.upload(P->sigma.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_eps.upload(P->eps.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_s6f.upload(P->s6f.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_floor2f.upload(P->floor2f.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_near2f.upload(P->near2f.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
+    delete P;
+    return hip_fail(e, "plan upload");
+  }
+  *out = P;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in CSR entry point state (shared by all callers, serialized).
+// ---------------------------------------------------------------------------
+struct LegacyCtx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  DevBuf<double> centers, axes, images, offsets, c12, c6, shift, floor2, output;
+  DevBuf<long long> starts;
+  DevBuf<int> indices, bad;
+};
+std::mutex g_legacy_mutex;
+LegacyCtx* g_legacy = nullptr;
+
+}  // namespace
+}  // namespace sv
+
+using namespace sv;
+
+extern "C" {
+
+const char* sv_last_error(void) { return t_error.c_str(); }
+
+int sv_abi_version(void) { return 1; }
+
+int sv_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int evaluate_reused_neighbors(int n_centers, int n_axes, int n_images, double cutoff2,
+                              const double* centers, const double* axes, const double* images,
+                              const double* offsets, const double* coeff12, const double* coeff6,
+                              const double* shift, const double* floor2, const int64_t* starts,
+                              const int32_t* indices, double* output) {
+  if (n_centers < 0 || n_axes < 1 || n_images < 1 || cutoff2 <= 0) return 1;
+  if (n_centers == 0) return 0;
+  // Host validation in the reference's loop order: the first center whose
+  // starts decrease (2) or whose range holds an out-of-range index (3) stops
+  // the evaluation; earlier centers are computed.
+  int stop = n_centers, stop_code = 0;
+  for (int c = 0; c < n_centers; ++c) {
+    if (starts[c] > starts[c + 1] || starts[c] < 0) {
+      stop = c, stop_code = 2;
+      break;
+    }
+    bool bad_index = false;
+    for (int64_t n = starts[c]; n < starts[c + 1]; ++n)
+      if (indices[n] < 0 || indices[n] >= n_images) {
+        bad_index = true;
+        break;
+      }
+    if (bad_index) {
+      stop = c, stop_code = 3;
+      break;
+    }
+  }
+  if (stop == 0) return stop_code;
+  std::lock_guard<std::mutex> lock(g_legacy_mutex);
+  if (!g_legacy) g_legacy = new LegacyCtx();
+  LegacyCtx& L = *g_legacy;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(5, "no HIP device");
+  if (L.device != dev) {
+    if (L.stream) (void)hipStreamDestroy(L.stream);
+    L.stream = nullptr;
+    L.device = dev;
+    if (hipStreamCreate(&L.stream) != hipSuccess) return fail(5, "hipStreamCreate failed");
+  }
+  const int64_t nnz = starts[stop];
+  hipStream_t s = L.stream;
+  SV_HIP(L.centers.upload(centers, 3 * (size_t)stop, s));
+  SV_HIP(L.axes.upload(axes, 3 * (size_t)n_axes, s));
+  SV_HIP(L.images.upload(images, 3 * (size_t)n_images, s));
+  SV_HIP(L.offsets.upload(offsets, 2, s));
+  SV_HIP(L.c12.upload(coeff12, n_images, s));
+  SV_HIP(L.c6.upload(coeff6, n_images, s));
+  SV_HIP(L.shift.upload(shift, n_images, s));
+  SV_HIP(L.floor2.upload(floor2, n_images, s));
+  SV_HIP(L.starts.upload((const long long*)starts, (size_t)stop + 1, s));
+  SV_HIP(L.indices.upload(indices, (size_t)std::max<int64_t>(nnz, 1), s));
+  SV_HIP(L.output.reserve((size_t)stop * n_axes, true));
+  SV_HIP(L.bad.reserve(1));
+  const int no_bad = 0x7fffffff;
+  SV_HIP(hipMemcpyAsync(L.bad.p, &no_bad, sizeof(int), hipMemcpyHostToDevice, s));
+  SV_HIP(launch_legacy(stop, n_axes, cutoff2, L.centers.p, L.axes.p, L.images.p, L.offsets.p,
+                       L.c12.p, L.c6.p, L.shift.p, L.floor2.p, L.starts.p, L.indices.p,
+                       L.output.p, L.bad.p, s));
+  int bad_center = no_bad;
+  SV_HIP(hipMemcpyAsync(output, L.output.p, (size_t)stop * n_axes * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+  SV_HIP(hipMemcpyAsync(&bad_center, L.bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  SV_HIP(hipStreamSynchronize(s));
+  if (bad_center < stop) return 4;
+  return stop_code;
+}
+
+int sv_plan_create(const double* cell, int n_atoms, const double* frac, const double* sigma,
+                   const double* epsilon, double guest_sigma, double guest_epsilon,
+                   const double* offsets, double cutoff, double near_energy_K, void** plan) {
+  if (!plan) return fail(1, "null output handle");
+  Plan* P = nullptr;
+  int rc = plan_create(cell, n_atoms, frac, sigma, epsilon, guest_sigma, guest_epsilon, offsets,
+                       cutoff, near_energy_K, &P);
+  *plan = P;
+  return rc;
+}
+
+int sv_plan_destroy(void* plan) {
+  delete static_cast<Plan*>(plan);
+  return 0;
+}
+
+int sv_plan_info(void* plan, int* n_types, double* keep_radius) {
+  if (!plan) return fail(1, "null plan");
+  Plan* P = static_cast<Plan*>(plan);
+  if (n_types) *n_types = P->n_types;
+  if (keep_radius) *keep_radius = P->keep_r;
+  return 0;
+}
+
+int sv_poses_create(void* plan, int n_centers, const double* centers_frac, int n_axes,
+                    const double* axes, void** poses) {
+  if (!plan || !poses || (n_centers > 0 && !centers_frac) || !axes) return fail(1, "null argument");
+  Poses* S = nullptr;
+  int rc = build_poses(static_cast<Plan*>(plan), n_centers, centers_frac, n_axes, axes, 0, &S);
+  *poses = S;
+  return rc;
+}
+
+int sv_poses_grid(void* plan, int size, int n_axes, const double* axes, void** poses) {
+  if (!plan || !poses || !axes) return fail(1, "null argument");
+  if (size < 1 || (long long)size * size * size > 0x7fffffffLL) return fail(1, "invalid grid size");
+  const long long n = (long long)size * size * size;
+  std::vector<double> f(3 * (size_t)n);
+  for (int ia = 0; ia < size; ++ia)
+    for (int ib = 0; ib < size; ++ib)
+      for (int ic = 0; ic < size; ++ic) {
+        const size_t i = ((size_t)ia * size + ib) * size + ic;
+        f[3 * i] = (ia + 0.5) / size;
+        f[3 * i + 1] = (ib + 0.5) / size;
+        f[3 * i + 2] = (ic + 0.5) / size;
+      }
+  Poses* S = nullptr;
+  int rc = build_poses(static_cast<Plan*>(plan), (int)n, f.data(), n_axes, axes, size, &S);
+  *poses = S;
+  return rc;
+}
+
+int sv_poses_destroy(void* poses) {
+  delete static_cast<Poses*>(poses);
+  return 0;
+}
+
+int sv_batch_create(int n_jobs, void* const* plans, void* const* poses, int precision,
+                    const double* temperatures, double* const* energy, float* const* orientation,
+                    float* const* marginal, float* const* grid, float* const* site, void** batch) {
+  if (!plans || !poses || !batch) return fail(1, "null argument");
+  Batch* B = nullptr;
+  int rc = batch_build(n_jobs, reinterpret_cast<Plan* const*>(plans),
+                       reinterpret_cast<Poses* const*>(poses), precision, temperatures, energy,
+                       orientation, marginal, grid, site, &B);
+  *batch = B;
+  return rc;
+}
+
+int sv_batch_run(void* batch, int time_kernels, int sync) {
+  if (!batch) return fail(1, "null batch");
+  return batch_run(static_cast<Batch*>(batch), time_kernels != 0, sync != 0);
+}
+
+int sv_batch_sync(void* batch) {
+  if (!batch) return fail(1, "null batch");
+  Batch* B = static_cast<Batch*>(batch);
+  SV_HIP(hipSetDevice(B->device));
+  int bad = 0;
+  SV_HIP(hipMemcpyAsync(&bad, B->d_bad.p, sizeof(int), hipMemcpyDeviceToHost, B->stream));
+  SV_HIP(hipStreamSynchronize(B->stream));
+  if (bad) return fail(4, "nonfinite Vext energy");
+  return 0;
+}
+
+int sv_batch_timings(void* batch, double* ms7) {
+  if (!batch || !ms7) return fail(1, "null argument");
+  Batch* B = static_cast<Batch*>(batch);
+  if (!B->timed_last) return fail(1, "last run was not timed");
+  SV_HIP(hipEventSynchronize(B->ev[8]));
+  const int pairs[6][2] = {{0, 1}, {1, 2}, {3, 4}, {5, 6}, {6, 7}, {7, 8}};
+  for (int i = 0; i < 6; ++i) {
+    float ms = 0.f;
+    SV_HIP(hipEventElapsedTime(&ms, B->ev[pairs[i][0]], B->ev[pairs[i][1]]));
+    ms7[i] = ms;
+  }
+  float all = 0.f;
+  SV_HIP(hipEventElapsedTime(&all, B->ev[0], B->ev[8]));
+  ms7[6] = all;
+  return 0;
+}
+
+int sv_batch_census(void* batch, unsigned long long* pairs_per_job) {
+  if (!batch || !pairs_per_job) return fail(1, "null argument");
+  Batch* B = static_cast<Batch*>(batch);
+  SV_HIP(hipSetDevice(B->device));
+  int rc = batch_lists(B, false);
+  if (rc) return rc;
+  SV_HIP(launch_zero_u64(B->d_pairs.p, B->n_jobs, B->stream));
+  SV_HIP(launch_census(B->d_jobs.p, B->n_jobs, B->d_job_work.p, B->n_work, B->d_offsets.p,
+                       B->d_list64.p, B->stream));
+  SV_HIP(hipMemcpyAsync(pairs_per_job, B->d_pairs.p, sizeof(unsigned long long) * B->n_jobs,
+                        hipMemcpyDeviceToHost, B->stream));
+  SV_HIP(hipStreamSynchronize(B->stream));
+  return 0;
+}
+
+int sv_batch_destroy(void* batch) {
+  delete static_cast<Batch*>(batch);
+  return 0;
+}
+
+int sv_field_run(void* plan, void* poses, int precision, double temperature, double* energy,
+                 float* orientation, float* marginal, float* grid, float* site) {
+  if (!plan || !poses) return fail(1, "null argument");
+  Plan* P = static_cast<Plan*>(plan);
+  Poses* S = static_cast<Poses*>(poses);
+  SV_HIP(hipSetDevice(P->device));
+  const size_t nc = (size_t)S->n_centers, np = nc * S->n_axes;
+  double* d_e = nullptr;
+  float *d_o = nullptr, *d_m = nullptr, *d_g = nullptr, *d_s = nullptr;
+  const bool need64 = energy || (marginal && S->n_groups > 1);
+  if (need64) {
+    SV_HIP(S->w_out64.reserve(np));
+    d_e = S->w_out64.p;
+  }
+  if (orientation) {
+    SV_HIP(S->w_out32.reserve(np));
+    d_o = S->w_out32.p;
+  }
+  if (marginal || grid) {
+    SV_HIP(S->w_marg.reserve(nc));
+    d_m = S->w_marg.p;
+  }
+  if (grid) {
+    SV_HIP(S->w_grid.reserve(nc));
+    d_g = S->w_grid.p;
+  }
+  if (site) {
+    SV_HIP(S->w_site.reserve(nc));
+    d_s = S->w_site.p;
+  }
+  Batch* B = nullptr;
+  void* pl = plan;
+  void* po = poses;
+  int rc = batch_build(1, reinterpret_cast<Plan* const*>(&pl), reinterpret_cast<Poses* const*>(&po),
+                       precision, &temperature, &d_e, &d_o, &d_m, &d_g, &d_s, &B);
+  if (rc) return rc;
+  rc = batch_run(B, false, true);
+  const hipStream_t s = B->stream;
+  if (rc == 0 || rc == 4) {
+    hipError_t e = hipSuccess;
+    if (energy && e == hipSuccess) e = hipMemcpyAsync(energy, d_e, np * 8, hipMemcpyDeviceToHost, s);
+    if (orientation && e == hipSuccess) e = hipMemcpyAsync(orientation, d_o, np * 4, hipMemcpyDeviceToHost, s);
+    if (marginal && e == hipSuccess) e = hipMemcpyAsync(marginal, d_m, nc * 4, hipMemcpyDeviceToHost, s);
+    if (grid && e == hipSuccess) e = hipMemcpyAsync(grid, d_g, nc * 4, hipMemcpyDeviceToHost, s);
+    if (site && e == hipSuccess) e = hipMemcpyAsync(site, d_s, nc * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "field download");
+  }
+  delete B;
+  return rc;
+}
+
+int sv_field_census(void* plan, void* poses, unsigned long long* pairs) {
+  if (!plan || !poses || !pairs) return fail(1, "null argument");
+  Batch* B = nullptr;
+  void* pl = plan;
+  void* po = poses;
+  int rc = batch_build(1, reinterpret_cast<Plan* const*>(&pl), reinterpret_cast<Poses* const*>(&po),
+                       PREC_FP64, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &B);
+  if (rc) return rc;
+  rc = sv_batch_census(B, pairs);
+  delete B;
+  return rc;
+}
+
+int sv_device_malloc(void** ptr, uint64_t bytes) {
+  if (!ptr) return fail(1, "null argument");
+  SV_HIP(hipMalloc(ptr, std::max<uint64_t>(bytes, 1)));
+  return 0;
+}
+
+int sv_device_free(void* ptr) {
+  if (ptr) SV_HIP(hipFree(ptr));
+  return 0;
+}
+
+int sv_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
+  SV_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int sv_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
+  SV_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // extern "C"

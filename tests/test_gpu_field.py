@@ -1,0 +1,154 @@
+"""GPU parity of the Vext field path against the FP64 oracle and the
+reference kernel's golden outputs (calls go through libsv_vext.so's C ABI)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests import _golden
+
+pytestmark = pytest.mark.gpu
+
+_P = ctypes.c_void_p
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from string_vext_amd import _lib
+    return _lib.load()
+
+
+@pytest.fixture(scope="module")
+def small_case():
+    from string_vext_amd import synthetic
+    return synthetic.small_field_case()
+
+
+def _legacy(lib, setup, centers_cart, axes, starts, indices, n_images=None, cutoff2=None):
+    out = np.full((len(centers_cart), len(axes)), np.nan)
+    arrays = [np.ascontiguousarray(centers_cart, dtype=np.float64),
+              np.ascontiguousarray(axes, dtype=np.float64), np.ascontiguousarray(setup.images),
+              np.ascontiguousarray(setup.offsets), np.ascontiguousarray(setup.coeff12),
+              np.ascontiguousarray(setup.coeff6), np.ascontiguousarray(setup.shift),
+              np.ascontiguousarray(setup.floor2), np.ascontiguousarray(starts, dtype=np.int64),
+              np.ascontiguousarray(indices, dtype=np.int32), out]
+    rc = lib.evaluate_reused_neighbors(
+        len(centers_cart), len(axes), len(setup.images) if n_images is None else n_images,
+        setup.cutoff ** 2 if cutoff2 is None else cutoff2,
+        *[a.ctypes.data_as(_P) for a in arrays])
+    return rc, out
+
+
+@pytest.mark.parametrize("name", _golden.NAMES)
+def test_dropin_abi_reproduces_reference_kernel_bits(oracle, lib, name):
+    """evaluate_reused_neighbors on the GPU == the reference kernel.cpp, bit for bit."""
+    case, centers, axes, ref = _golden.load(name)
+    setup = oracle.ReuseSetup(case)
+    axes = axes / np.linalg.norm(axes, axis=1, keepdims=True)
+    cart = np.mod(centers, 1) @ setup.cell
+    starts, indices = oracle.csr(setup.tree, cart, setup.radius)
+    rc, out = _legacy(lib, setup, cart, axes, starts, indices)
+    assert rc == 0
+    assert np.array_equal(out, ref), np.abs(out - ref).max()
+
+
+def test_dropin_abi_matches_oracle_on_small_field(oracle, lib, small_case):
+    setup = oracle.ReuseSetup(small_case)
+    centers, axes = oracle.regular_pose_grid(6, 64)
+    cart = np.mod(centers, 1) @ setup.cell
+    starts, indices = oracle.csr(setup.tree, cart, setup.radius)
+    rc, out = _legacy(lib, setup, cart, axes, starts, indices)
+    assert rc == 0
+    ref = oracle.run_reuse_kernel(oracle.lib().oracle_reuse, cart, axes, setup, starts, indices)
+    assert np.array_equal(out, ref)
+
+
+def test_dropin_abi_error_codes(oracle, lib):
+    case, centers, axes, ref = _golden.load("toy")
+    setup = oracle.ReuseSetup(case)
+    axes = axes / np.linalg.norm(axes, axis=1, keepdims=True)
+    cart = np.mod(centers, 1) @ setup.cell
+    starts, indices = oracle.csr(setup.tree, cart, setup.radius)
+    # code 1: invalid sizes / cutoff (kernel.cpp:11-12)
+    assert _legacy(lib, setup, cart, axes, starts, indices, cutoff2=0.0)[0] == 1
+    assert _legacy(lib, setup, cart, axes, starts, indices, n_images=0)[0] == 1
+    assert _legacy(lib, setup, cart, axes[:0], starts, indices)[0] == 1
+    # code 2: decreasing starts at center 5; earlier rows computed (kernel.cpp:14-15)
+    bad = starts.copy()
+    bad[6] = bad[5] - 1
+    rc, out = _legacy(lib, setup, cart, axes, bad, indices)
+    assert rc == 2
+    np.testing.assert_array_equal(out[:5], ref[:5])
+    # code 3: out-of-range index inside center 7's range (kernel.cpp:24-25)
+    bad_idx = indices.copy()
+    bad_idx[starts[7]] = len(setup.images)
+    rc, out = _legacy(lib, setup, cart, axes, starts, bad_idx)
+    assert rc == 3
+    np.testing.assert_array_equal(out[:7], ref[:7])
+    # code 4: nonfinite energy (kernel.cpp:39-40)
+    setup.coeff12 = setup.coeff12.copy()
+    setup.coeff12[indices[starts[3]:starts[4]]] = np.inf
+    assert _legacy(lib, setup, cart, axes, starts, indices)[0] == 4
+
+
+@pytest.mark.parametrize("name", _golden.NAMES)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_field_matches_reference_golden(name, precision):
+    from string_vext_amd.field import Plan
+    case, centers, axes, ref = _golden.load(name)
+    axes = axes / np.linalg.norm(axes, axis=1, keepdims=True)
+    plan = Plan.from_case(case)
+    out = plan.field(plan.poses(centers, axes), precision=precision)["energy"]
+    rel = np.abs(out - ref) / np.maximum(np.abs(ref), 1.0)
+    # FP64: the plain GPU form within 1e-6 relative (1 K floor) of the reference CPU
+    # kernel -- in practice rounding-level; mixed: the stated mixed tolerance
+    assert rel.max() < (1e-12 if precision == "fp64" else 1e-5), rel.max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_small_field_all_channels_match_oracle(oracle, small_case, precision):
+    from string_vext_amd.field import Plan
+    size, k = 8, 64
+    ref = oracle.vext_fields(small_case, size, k)
+    centers, axes = oracle.regular_pose_grid(size, k)
+    plan = Plan.from_case(small_case)
+    out = plan.field(plan.grid(size, axes), precision=precision, orientation=True,
+                     marginal=True, grid=True, site=True,
+                     temperature=small_case["temperature"])
+    m = oracle.error_metrics(ref["energy"], out["energy"])
+    if precision == "fp64":
+        assert m["all_max_relative_error_floor1K"] < 1e-12
+    else:
+        assert m["accessible_max_abs_error_K"] < 1e-3
+        assert m["all_max_relative_error_floor1K"] < 1e-5
+    np.testing.assert_allclose(out["orientation"].reshape(ref["orientation_K"].shape),
+                               ref["orientation_K"], rtol=1e-6, atol=1e-3)
+    np.testing.assert_allclose(out["marginal"].reshape(ref["marginal_K"].shape),
+                               ref["marginal_K"], rtol=1e-6, atol=1e-3)
+    np.testing.assert_allclose(out["grid"].reshape(ref["grid"].shape), ref["grid"],
+                               rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(out["site"].reshape(ref["site_K"][0].shape), ref["site_K"][0],
+                               rtol=1e-6, atol=1e-3)
+
+
+def test_census_counts_exact_pairs(oracle, small_case):
+    from string_vext_amd.field import Plan
+    centers, axes = oracle.regular_pose_grid(6, 64)
+    plan = Plan.from_case(small_case)
+    assert plan.census(plan.grid(6, axes)) == oracle.count_pairs(small_case, centers, axes)
+
+
+def test_arbitrary_centers_and_many_axes(oracle, small_case):
+    """Random (unsorted) centers, 100 directors (> one wave of orientations)."""
+    from string_vext_amd.field import Plan
+    rng = np.random.default_rng(7)
+    centers = rng.uniform(-1.5, 2.5, size=(50, 3))
+    axes = rng.normal(size=(100, 3))
+    ref = oracle.reuse_energy(small_case, centers, axes)
+    plan = Plan.from_case(small_case)
+    out = plan.field(plan.poses(centers, axes / np.linalg.norm(axes, axis=1, keepdims=True)),
+                     precision="fp64", marginal=True, temperature=298.0)
+    rel = np.abs(out["energy"] - ref) / np.maximum(np.abs(ref), 1.0)
+    assert rel.max() < 1e-12
+    np.testing.assert_allclose(out["marginal"], oracle.marginal_energy(ref, 298.0),
+                               rtol=1e-6, atol=1e-3)
