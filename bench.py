@@ -43,18 +43,25 @@ def log(msg):
 
 
 def time_field(plan, poses, precision, steps, warmup, temperature):
+    """Seconds per field for `steps` back-to-back fields (synchronized before and
+    after the timed region), then the per-kernel durations from HIP events on
+    the batch's stream over a second pass of the same steps."""
     from string_vext_amd.field import Batch
     batch = Batch([(plan, poses)], precision=precision, temperatures=temperature)
     for _ in range(warmup):
         batch.run(time_kernels=False, sync=True)
-    field_ms, run_ms = [], []
+    batch.sync()
     t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(time_kernels=False, sync=False)
+    batch.sync()
+    wall = time.perf_counter() - t0
+    field_ms, run_ms = [], []
     for _ in range(steps):
         batch.run(time_kernels=True, sync=True)
         t = batch.timings()
         field_ms.append(t["field"])
         run_ms.append(t["run"])
-    wall = time.perf_counter() - t0
     timings = batch.timings()
     batch.close()
     return wall / steps, float(np.mean(field_ms)), float(np.mean(run_ms)), timings
@@ -129,6 +136,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="grid centers in the CPU baseline sample (0 skips it)")
     ap.add_argument("--no-check", action="store_true", help="skip the mixed-vs-FP64 error pass")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain elementwise copy of MIB MiB (FP32 in, FP32 out) so that "
+                         "PMC byte counters can be scaled against a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every pose is independent, "
@@ -174,6 +184,15 @@ def main():
         log(f"cpu baseline on {args.cpu_sample} centers ...")
         cpu = cpu_baseline(case, centers, axes, args.cpu_sample)
         log(f"cpu: {cpu['value']:.3e} pair-evals/s")
+
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {n * 4} bytes read, {n * 4} bytes written")
 
     value = pairs / step_s
     line = {

@@ -108,8 +108,6 @@ hipError_t launch_census(const Job* d_jobs, int n_jobs, const long long* d_job_w
                          hipStream_t s);
 hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job_first_center,
                            long long n_centers_total, hipStream_t s);
-hipError_t launch_site(const Job* d_jobs, const SiteChunk* d_chunks, int n_chunks,
-                       const long long* d_offsets, const double* d_list64, hipStream_t s);
 hipError_t launch_legacy(int n_centers, int n_axes, double cutoff2, const double* centers,
                          const double* axes, const double* images, const double* offsets,
                          const double* coeff12, const double* coeff6, const double* shift,

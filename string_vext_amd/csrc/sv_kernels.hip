@@ -234,6 +234,7 @@ struct WaveCtx {
   const Job* J;
   int slot;       // sorted center slot
   int tile;       // job-local tile
+  int group;      // orientation group (64 axes per group)
   int k;          // this lane's orientation
   bool k_ok;
 };
@@ -248,16 +249,18 @@ __device__ __forceinline__ bool setup_wave(const Job* __restrict__ jobs, int n_j
   const long long local = w - c.J->work_begin;
   const int ng = c.J->n_groups;
   c.slot = (int)(local / ng);
-  const int group = (int)(local - (long long)c.slot * ng);
+  c.group = (int)(local - (long long)c.slot * ng);
   c.tile = c.J->tile_of[c.slot];
   const int lane = threadIdx.x & 63;
-  c.k = group * 64 + lane;
+  c.k = c.group * 64 + lane;
   c.k_ok = c.k < c.J->n_axes;
   if (!c.k_ok) c.k = c.J->n_axes - 1;
   return true;
 }
 
-__device__ __forceinline__ void write_pose(const WaveCtx& c, double E) {
+// Pose energies, fused marginal/grid, and the site field at the center
+// (lane partials `site`, already including every shift term).
+__device__ __forceinline__ void write_pose(const WaveCtx& c, double E, bool with_site, double site) {
   const Job& J = *c.J;
   const int cid = J.center_id[c.slot];
   if (c.k_ok) {
@@ -279,6 +282,13 @@ __device__ __forceinline__ void write_pose(const WaveCtx& c, double E) {
       if (!isfinite(marg)) atomicOr(J.bad, 1);
     }
   }
+  if (with_site) {
+    const double v = wave_sum(site);
+    if ((threadIdx.x & 63) == 0) {
+      J.site32[cid] = (float)v;
+      if (!isfinite(v)) atomicOr(J.bad, 1);
+    }
+  }
 }
 
 // FP64 pair term with the reference's exact cutoff/floor semantics
@@ -298,6 +308,27 @@ __device__ __forceinline__ double pair64(double o, double ux, double uy, double 
   return e;
 }
 
+// 1/x in FP64 from v_rcp_f64 and two Newton steps (a few ulp; the mixed
+// kernel's near-contact terms only need FP64-level relative accuracy, not
+// IEEE rounding).
+__device__ __forceinline__ double inv_fast(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+// Same pair term from a precomputed squared distance (mixed near path).
+__device__ __forceinline__ double pair64_d2(double d2, double rc2, double c12, double c6,
+                                            double sh, double fl2) {
+  const double safe = d2 < fl2 ? fl2 : d2;
+  const double inv = inv_fast(safe);
+  const double i6 = inv * inv * inv;
+  const double e = fma(c12 * i6, i6, fma(-c6, i6, sh));
+  return d2 < rc2 ? e : 0.0;
+}
+
 template <bool SYM>
 __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict__ jobs, int n_jobs,
                                                           const long long* __restrict__ job_work,
@@ -312,6 +343,10 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
   WaveCtx c;
   if (!setup_wave(jobs, n_jobs, job_work, n_work, wv, c)) return;
   const Job& J = *c.J;
+  // The site field of a center is the same pair sum taken at the center
+  // itself over the same candidates; the group-0 wave of the center
+  // accumulates it lane-parallel while it compacts the candidate list.
+  const bool do_site = J.site32 != nullptr && c.group == 0;
 
   const double ux64 = J.axes64[3 * c.k], uy64 = J.axes64[3 * c.k + 1], uz64 = J.axes64[3 * c.k + 2];
   const float ux = (float)ux64, uy = (float)uy64, uz = (float)uz64;
@@ -320,14 +355,16 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
   const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
   const float o0sq = J.o0sqf, dR2 = J.dR2f;
-  const double o0 = J.o0, o1 = J.o1, rc2 = J.rc2;
+  const double rc2 = J.rc2;
+  const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
+  const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1;
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
 
-  double E = 0.0;
+  double E = 0.0, S = 0.0, S_shift = 0.0;
   for (int t = 0; t < J.n_types; ++t) {
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
-    const double c6 = J.c6[t];
+    const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
     long long kept_far = 0;
     for (long long base = beg; base < end; base += 64) {
       const long long idx = base + lane;
@@ -345,11 +382,25 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
       const unsigned long long mnear = __ballot(is_near);
       if (is_far) s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, rr + o0sq);
       if (is_near) s_near[wv][lanes_below(mnear)] = (int)(idx - base);
+      if (do_site) {
+        // far candidates: clamped FP32 term (the shift is added per kept entry below);
+        // near candidates: the FP64 reference term from the FP64 offsets.
+        const float ds = __builtin_amdgcn_fmed3f(rr, fl2f, rc2f);
+        const float is = __builtin_amdgcn_rcpf(ds);
+        const float is6 = is * is * is;
+        const float sf = is6 * fmaf(is6, s6, -1.0f);
+        S = fma(c6, (double)(is_far ? sf : 0.0f), S);
+        if (is_near) {
+          const double qx = list64[3 * idx] - cx, qy = list64[3 * idx + 1] - cy,
+                       qz = list64[3 * idx + 2] - cz;
+          S += pair64_d2(qx * qx + qy * qy + qz * qz, rc2, c12, c6, sh, fl2);
+        }
+      }
       wave_sync();
       const int nf = __popcll(mfar), nn = __popcll(mnear);
       kept_far += nf;
       float acc = 0.f;
-#pragma unroll 2
+#pragma unroll 4
       for (int q = 0; q < nf; ++q) {
         const float4 f = s_far[wv][q];
         const float tdot = f.x * ux + f.y * uy + f.z * uz;
@@ -365,22 +416,23 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
         acc = fmaf(i16, fmaf(i16, s6, -1.0f), acc);
       }
       E = fma(c6, (double)acc, E);
-      if (nn) {
-        const double c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
-        for (int q = 0; q < nn; ++q) {
-          const long long id = base + s_near[wv][q];
-          const double rx64 = list64[3 * id] - cx, ry64 = list64[3 * id + 1] - cy,
-                       rz64 = list64[3 * id + 2] - cz;
-          E += pair64(o0, ux64, uy64, uz64, rx64, ry64, rz64, rc2, c12, J.c6[t], sh, fl2);
-          E += pair64(o1, ux64, uy64, uz64, rx64, ry64, rz64, rc2, c12, J.c6[t], sh, fl2);
-        }
+      for (int q = 0; q < nn; ++q) {
+        // FP64 near-contact pair: one dot product serves both sites.
+        const long long id = base + s_near[wv][q];
+        const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
+                     qz = list64[3 * id + 2] - cz;
+        const double RR = qx * qx + qy * qy + qz * qz;
+        const double tt = qx * ux64 + qy * uy64 + qz * uz64;
+        E += pair64_d2(fma(tt, m2o0d, RR + o0sqd), rc2, c12, c6, sh, fl2);
+        E += pair64_d2(fma(tt, m2o1d, RR + o1sqd), rc2, c12, c6, sh, fl2);
       }
       wave_sync();
     }
     // Every kept far pair was clamped to rc^2 at worst, contributing -shift.
-    E = fma((double)(2 * kept_far), J.shift[t], E);
+    E = fma((double)(2 * kept_far), sh, E);
+    S_shift = fma((double)kept_far, sh, S_shift);
   }
-  write_pose(c, E);
+  write_pose(c, E, do_site, lane == 0 ? S + S_shift : S);
 }
 
 __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__ jobs, int n_jobs,
@@ -394,12 +446,13 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
   WaveCtx c;
   if (!setup_wave(jobs, n_jobs, job_work, n_work, wv, c)) return;
   const Job& J = *c.J;
+  const bool do_site = J.site32 != nullptr && c.group == 0;
   const double ux = J.axes64[3 * c.k], uy = J.axes64[3 * c.k + 1], uz = J.axes64[3 * c.k + 2];
   const double cx = J.center_rel[3 * c.slot], cy = J.center_rel[3 * c.slot + 1],
                cz = J.center_rel[3 * c.slot + 2];
   const double keep2 = J.keep_r * J.keep_r, rc2 = J.rc2, o0 = J.o0, o1 = J.o1;
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
-  double E = 0.0;
+  double E = 0.0, S = 0.0;
   for (int t = 0; t < J.n_types; ++t) {
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
     const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
@@ -415,6 +468,7 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
       if (keep) {
         const int p = lanes_below(m);
         s_r[wv][0][p] = rx; s_r[wv][1][p] = ry; s_r[wv][2][p] = rz;
+        if (do_site) S += pair64(0.0, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2);
       }
       wave_sync();
       const int n = __popcll(m);
@@ -426,7 +480,7 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
       wave_sync();
     }
   }
-  write_pose(c, E);
+  write_pose(c, E, do_site, S);
 }
 
 // Exact count of in-cutoff (site, image) pairs: the pair-eval census that the
@@ -503,46 +557,6 @@ __global__ __launch_bounds__(64) void marginal_kernel(const Job* __restrict__ jo
     J.marg32[cid] = mf;
     if (J.grid32) J.grid32[cid] = asinhf(mf / 1000.0f);
     if (!isfinite(marg)) atomicOr(J.bad, 1);
-  }
-}
-
-// Single-site field at every center (reference physics.py:59-85 with the
-// case cutoff): r = |d|, ratio = (sigma / max(r, 0.1 sigma))^6,
-// 4 eps (ratio^2 - ratio) + shift for r < rc.  Lanes are member centers of
-// one tile; the tile list is read by broadcast.
-__global__ __launch_bounds__(64) void site_kernel(const Job* __restrict__ jobs,
-                                                  const SiteChunk* __restrict__ chunks,
-                                                  const long long* __restrict__ offsets,
-                                                  const double* __restrict__ list64) {
-  const SiteChunk ch = chunks[blockIdx.x];
-  const Job& J = jobs[ch.job];
-  const int lane = threadIdx.x;
-  const bool ok = lane < ch.count;
-  const int slot = ch.first + (ok ? lane : 0);
-  const double cx = J.center_rel[3 * slot], cy = J.center_rel[3 * slot + 1],
-               cz = J.center_rel[3 * slot + 2];
-  const long long seg0 = J.seg_base + (long long)ch.tile * J.n_types;
-  const double rc2 = J.rc2;
-  double E = 0.0;
-  for (int t = 0; t < J.n_types; ++t) {
-    const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
-    const double sg = J.sigma[t], e4 = 4.0 * J.eps[t], sh = J.shift[t];
-    for (long long i = beg; i < end; ++i) {
-      const double dx = list64[3 * i] - cx, dy = list64[3 * i + 1] - cy,
-                   dz = list64[3 * i + 2] - cz;
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      if (d2 < rc2) {
-        const double r = sqrt(d2);
-        const double q = sg / fmax(r, 0.1 * sg);
-        const double q2 = q * q;
-        const double ratio = q2 * q2 * q2;
-        E += e4 * (ratio * ratio - ratio) + sh;
-      }
-    }
-  }
-  if (ok) {
-    J.site32[J.center_id[slot]] = (float)E;
-    if (!isfinite(E)) atomicOr(J.bad, 1);
   }
 }
 
@@ -683,14 +697,6 @@ hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job
   if (n_total <= 0) return hipSuccess;
   hipLaunchKernelGGL(marginal_kernel, dim3((unsigned)n_total), dim3(64), 0, s, d_jobs, n_jobs,
                      d_job_first, n_total);
-  return hipGetLastError();
-}
-
-hipError_t launch_site(const Job* d_jobs, const SiteChunk* d_chunks, int n_chunks,
-                       const long long* d_offsets, const double* d_list64, hipStream_t s) {
-  if (n_chunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(site_kernel, dim3(n_chunks), dim3(64), 0, s, d_jobs, d_chunks, d_offsets,
-                     d_list64);
   return hipGetLastError();
 }
 

@@ -453,9 +453,7 @@ int batch_run(Batch* B, bool timed, bool sync) {
   if (B->need_marginal)
     SV_HIP(launch_marginal(B->d_jobs.p, B->n_jobs, B->d_job_first.p, B->n_centers_total, B->stream));
   if (timed) SV_HIP(hipEventRecord(B->ev[7], B->stream));
-  if (B->any_site)
-    SV_HIP(launch_site(B->d_jobs.p, B->d_chunks.p, (int)B->chunks.size(), B->d_offsets.p,
-                       B->d_list64.p, B->stream));
+  // (the site field is fused into the field kernel)
   if (timed) SV_HIP(hipEventRecord(B->ev[8], B->stream));
   B->timed_last = timed;
   if (!sync) return 0;
