@@ -35,8 +35,9 @@ struct Job {
   float m2o0f, m2o1f;   // -2*o0, -2*o1
   float o0sqf;          // o0^2
   float dR2f;           // o1^2 - o0^2 (0 for a symmetric guest)
+  float in2f;           // (rc - max|o| - margin)^2: closer atoms are inside the
+                        // cutoff for every orientation
   int sym;              // o1 == -o0 exactly
-  int pad0;
   // ---- per-type coefficients (index = framework LJ type) ------------------
   const double* c12;    // 4 eps sigma^12          (mixed Lorentz-Berthelot)
   const double* c6;     // 4 eps sigma^6
@@ -48,6 +49,8 @@ struct Job {
   const float* floor2f; // floor2 in FP32
   const float* near2f;  // (near radius)^2: center-atom distance below which the
                         // mixed kernel evaluates the pair in FP64
+  const float* lo2f;    // (sqrt(floor2) + max|o| + margin)^2: beyond it no
+                        // orientation reaches the contact floor
   // ---- pose set -----------------------------------------------------------
   const double* center_rel;  // [n_centers][3] Cartesian offset from tile centre (sorted)
   const int* center_id;      // sorted slot -> caller's center index

@@ -336,6 +336,7 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
                                                           const long long* __restrict__ offsets,
                                                           const float4* __restrict__ list32,
                                                           const double* __restrict__ list64) {
+  __shared__ float4 s_in[4][64];
   __shared__ float4 s_far[4][64];
   __shared__ int s_near[4][64];
   const int lane = threadIdx.x & 63;
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
                cz = J.center_rel[3 * c.slot + 2];
   const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
-  const float o0sq = J.o0sqf, dR2 = J.dR2f;
+  const float o0sq = J.o0sqf, dR2 = J.dR2f, in2 = J.in2f;
   const double rc2 = J.rc2;
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
   const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1;
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
   double E = 0.0, S = 0.0, S_shift = 0.0;
   for (int t = 0; t < J.n_types; ++t) {
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
-    const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
+    const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t], lo2 = J.lo2f[t];
     const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
     long long kept_far = 0;
     for (long long base = beg; base < end; base += 64) {
@@ -377,9 +378,14 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
       }
       const bool keep = rr < keep2f;
       const bool is_near = keep && rr < near2;
-      const bool is_far = keep && !is_near;
+      // inner: every orientation puts both sites strictly between the contact
+      // floor and the cutoff, so the pair needs no clamp
+      const bool is_in = keep && !is_near && rr < in2 && rr > lo2;
+      const bool is_far = keep && !is_near && !is_in;
+      const unsigned long long min_ = __ballot(is_in);
       const unsigned long long mfar = __ballot(is_far);
       const unsigned long long mnear = __ballot(is_near);
+      if (is_in) s_in[wv][lanes_below(min_)] = make_float4(rx, ry, rz, rr + o0sq);
       if (is_far) s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, rr + o0sq);
       if (is_near) s_near[wv][lanes_below(mnear)] = (int)(idx - base);
       if (do_site) {
@@ -389,7 +395,7 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
         const float is = __builtin_amdgcn_rcpf(ds);
         const float is6 = is * is * is;
         const float sf = is6 * fmaf(is6, s6, -1.0f);
-        S = fma(c6, (double)(is_far ? sf : 0.0f), S);
+        S = fma(c6, (double)(is_far || is_in ? sf : 0.0f), S);
         if (is_near) {
           const double qx = list64[3 * idx] - cx, qy = list64[3 * idx + 1] - cy,
                        qz = list64[3 * idx + 2] - cz;
@@ -397,9 +403,22 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
         }
       }
       wave_sync();
-      const int nf = __popcll(mfar), nn = __popcll(mnear);
-      kept_far += nf;
-      float acc = 0.f;
+      const int ni = __popcll(min_), nf = __popcll(mfar), nn = __popcll(mnear);
+      kept_far += ni + nf;
+      float acc = 0.f, acc1 = 0.f;
+#pragma unroll 4
+      for (int q = 0; q < ni; ++q) {
+        const float4 f = s_in[wv][q];
+        const float tdot = f.x * ux + f.y * uy + f.z * uz;
+        const float d0 = fmaf(tdot, m2o0, f.w);
+        const float d1 = fmaf(tdot, m2o1, SYM ? f.w : f.w + dR2);
+        const float i0 = __builtin_amdgcn_rcpf(d0);
+        const float i1 = __builtin_amdgcn_rcpf(d1);
+        const float i06 = i0 * i0 * i0;
+        const float i16 = i1 * i1 * i1;
+        acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
+        acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
+      }
 #pragma unroll 4
       for (int q = 0; q < nf; ++q) {
         const float4 f = s_far[wv][q];
@@ -413,9 +432,9 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
         const float i06 = i0 * i0 * i0;
         const float i16 = i1 * i1 * i1;
         acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
-        acc = fmaf(i16, fmaf(i16, s6, -1.0f), acc);
+        acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
       }
-      E = fma(c6, (double)acc, E);
+      E = fma(c6, (double)(acc + acc1), E);
       for (int q = 0; q < nn; ++q) {
         // FP64 near-contact pair: one dot product serves both sites.
         const long long id = base + s_near[wv][q];

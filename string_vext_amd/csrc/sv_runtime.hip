@@ -103,11 +103,11 @@ struct Plan {
   std::vector<double> frac;       // sorted by type, wrapped
   std::vector<int> type_start;
   std::vector<double> c12, c6, shift, floor2, sigma, eps;
-  std::vector<float> s6f, floor2f, near2f;
+  std::vector<float> s6f, floor2f, near2f, lo2f;
   double o0, o1, cutoff, keep_r, near_energy;
   DevBuf<double> d_frac, d_c12, d_c6, d_shift, d_floor2, d_sigma, d_eps;
   DevBuf<int> d_type_start;
-  DevBuf<float> d_s6f, d_floor2f, d_near2f;
+  DevBuf<float> d_s6f, d_floor2f, d_near2f, d_lo2f;
   ~Plan() {
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -312,6 +312,10 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
     J.m2o1f = (float)(-2.0 * P->o1);
     J.o0sqf = (float)(P->o0 * P->o0);
     J.dR2f = (float)(P->o1 * P->o1 - P->o0 * P->o0);
+    {
+      const double in_r = P->cutoff - std::max(std::fabs(P->o0), std::fabs(P->o1)) - 1e-3;
+      J.in2f = in_r > 0.0 ? (float)(in_r * in_r) : 0.0f;
+    }
     J.sym = (P->o1 == -P->o0);
     if (!J.sym) B->all_sym = false;
     J.c12 = P->d_c12.p;
@@ -323,6 +327,7 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
     J.s6f = P->d_s6f.p;
     J.floor2f = P->d_floor2f.p;
     J.near2f = P->d_near2f.p;
+    J.lo2f = P->d_lo2f.p;
     J.center_rel = S->d_center_rel.p;
     J.center_id = S->d_center_id.p;
     J.tile_of = S->d_tile_of.p;
@@ -556,6 +561,8 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
       near2 = (float)(rn * rn);
     }
     P->near2f.push_back(near2);
+    const double lo = std::sqrt(fl) + omax + 1e-3;
+    P->lo2f.push_back((float)(lo * lo));
   }
   hipError_t e;
   if ((e = hipGetDevice(&P->device)) != hipSuccess || (e = hipStreamCreate(&P->stream)) != hipSuccess ||
@@ -570,6 +577,7 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
       (e = P->d_s6f.upload(P->s6f.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = P->d_floor2f.upload(P->floor2f.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = P->d_near2f.upload(P->near2f.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_lo2f.upload(P->lo2f.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
     delete P;
     return hip_fail(e, "plan upload");
