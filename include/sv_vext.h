@@ -129,6 +129,24 @@ int sv_batch_timings(void *batch, double *ms7);
 int sv_batch_census(void *batch, unsigned long long *pairs_per_job);
 int sv_batch_destroy(void *batch);
 
+/* ------------------------------------------------------------------------ */
+/* Independent poses: pose i is (center_i, director_i); used for energies at */
+/* arbitrary poses and their gradients (torch.autograd, string images).  No  */
+/* counterpart in the reference, whose String forces are forward differences */
+/* of the energy (native/string/source_code/global.h:1281-1324).             */
+/*   centers_cart [n][3] Cartesian (wrapped internally), axes [n][3] used as  */
+/*   given (callers normalize).                                              */
+/* sv_pose_list_eval writes DEVICE arrays (each may be NULL):                */
+/*   energy [n] FP64; grad_center [n][3] = g_i dE_i/dc_i;                     */
+/*   grad_axis [n][3] = g_i dE_i/du_i, with g = grad_out [n] (device) or 1.    */
+/* The backward pass recomputes every pair term (nothing is stored).         */
+/* Synchronous; returns 4 on non-finite results.                             */
+/* ------------------------------------------------------------------------ */
+int sv_pose_list_create(void *plan, int n_poses, const double *centers_cart, const double *axes,
+                        void **poses);
+int sv_pose_list_eval(void *poses, int precision, const double *grad_out, double *energy,
+                      double *grad_center, double *grad_axis);
+
 /* Device memory helpers for callers without their own allocator. */
 int sv_device_malloc(void **ptr, uint64_t bytes);
 int sv_device_free(void *ptr);

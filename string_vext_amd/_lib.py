@@ -38,6 +38,8 @@ SIGNATURES = [
     ("sv_batch_timings", _I, [_P, _P]),
     ("sv_batch_census", _I, [_P, _P]),
     ("sv_batch_destroy", _I, [_P]),
+    ("sv_pose_list_create", _I, [_P, _I, _P, _P, _P]),
+    ("sv_pose_list_eval", _I, [_P, _I, _P, _P, _P, _P]),
     ("sv_device_malloc", _I, [_P, ctypes.c_uint64]),
     ("sv_device_free", _I, [_P]),
     ("sv_memcpy_d2h", _I, [_P, _P, ctypes.c_uint64]),

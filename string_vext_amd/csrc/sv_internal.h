@@ -75,6 +75,13 @@ struct Job {
   unsigned long long* pair_count;  // census output (pairs with d^2 < rc^2)
   int* bad;                  // set to 1 when a nonfinite energy is produced
   int pad2;
+  // ---- independent poses (one director per center) ------------------------
+  const double* pose_axes64; // [n_centers][3] director of each sorted slot
+  const float* pose_axes32;
+  const double* grad_out;    // [n_centers] upstream gradient by caller index (null = 1)
+  double* pose_e;            // [n_centers] FP64 energy by caller index
+  double* grad_c;            // [n_centers][3] dE/d(center), Cartesian
+  double* grad_u;            // [n_centers][3] dE/d(director), director used as given
 };
 
 // Global tile table entry (one per tile of every job).
@@ -111,6 +118,9 @@ hipError_t launch_census(const Job* d_jobs, int n_jobs, const long long* d_job_w
                          hipStream_t s);
 hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job_first_center,
                            long long n_centers_total, hipStream_t s);
+hipError_t launch_pose(int precision, bool grad, const Job* d_jobs, const SiteChunk* d_chunks,
+                       int n_chunks, const long long* d_offsets, const float4* d_list32,
+                       const double* d_list64, hipStream_t s);
 hipError_t launch_legacy(int n_centers, int n_axes, double cutoff2, const double* centers,
                          const double* axes, const double* images, const double* offsets,
                          const double* coeff12, const double* coeff6, const double* shift,

@@ -502,6 +502,162 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
   write_pose(c, E, do_site, S);
 }
 
+// ---------------------------------------------------------------------------
+// Independent poses (center_i, director_i): energy and, with GRAD, the
+// analytic gradient.  Lanes are up to 64 poses of one tile; every lane walks
+// the tile's whole candidate list (read by broadcast from LDS).  With
+// r = image - center, t = r.u and g_s = dphi/d(d^2) of site s,
+//   dE/dc = 2 [u sum_s o_s G_s - sum_j r_j (g_0j + g_1j)]
+//   dE/du = 2 [u sum_s o_s^2 G_s - sum_j r_j (o_0 g_0j + o_1 g_1j)]
+// where G_s = sum_j g_sj; g = 0 beyond the cutoff and inside the contact floor
+// (the floored term is constant there), as in the FP64 reference formula.
+// ---------------------------------------------------------------------------
+struct PoseAcc {
+  double E, G0, G1, Rc[3], Ru[3], Gc[3], Gu[3];
+};
+
+// FP64 pair energy and gradient contribution with explicit d = o u - r.
+__device__ __forceinline__ void pose_pair64(double o, double ux, double uy, double uz, double rx,
+                                            double ry, double rz, double rc2, double c12,
+                                            double c6, double sh, double fl2, bool grad,
+                                            PoseAcc& a) {
+  const double dx = o * ux - rx, dy = o * uy - ry, dz = o * uz - rz;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  if (d2 < rc2) {
+    const bool floored = d2 < fl2;
+    const double safe = floored ? fl2 : d2;
+    const double inv = 1.0 / safe;
+    const double i6 = inv * inv * inv;
+    a.E += c12 * i6 * i6 - c6 * i6 + sh;
+    if (grad && !floored) {
+      const double g = (-6.0 * c12 * i6 * i6 + 3.0 * c6 * i6) * inv;
+      a.Gc[0] += g * dx; a.Gc[1] += g * dy; a.Gc[2] += g * dz;
+      a.Gu[0] += g * o * dx; a.Gu[1] += g * o * dy; a.Gu[2] += g * o * dz;
+    }
+  }
+}
+
+template <bool MIXED, bool GRAD>
+__global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
+                                                  const SiteChunk* __restrict__ chunks,
+                                                  const long long* __restrict__ offsets,
+                                                  const float4* __restrict__ list32,
+                                                  const double* __restrict__ list64) {
+  __shared__ float4 s_e[64];
+  __shared__ double s_d[3][64];
+  const SiteChunk ch = chunks[blockIdx.x];
+  const Job& J = jobs[ch.job];
+  const int lane = threadIdx.x;
+  const bool ok = lane < ch.count;
+  const int slot = ch.first + (ok ? lane : 0);
+  const double cx = J.center_rel[3 * slot], cy = J.center_rel[3 * slot + 1],
+               cz = J.center_rel[3 * slot + 2];
+  const double ux = J.pose_axes64[3 * slot], uy = J.pose_axes64[3 * slot + 1],
+               uz = J.pose_axes64[3 * slot + 2];
+  const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
+  const float uxf = (float)ux, uyf = (float)uy, uzf = (float)uz;
+  const double o0 = J.o0, o1 = J.o1, rc2 = J.rc2;
+  const float rc2f = J.rc2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
+  const float o0sq = J.o0sqf, o1sq = J.o0sqf + J.dR2f;
+  const float o0f = (float)o0, o1f = (float)o1;
+  const long long seg0 = J.seg_base + (long long)ch.tile * J.n_types;
+  PoseAcc a = {};
+  for (int t = 0; t < J.n_types; ++t) {
+    const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
+    const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
+    const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
+    for (long long base = beg; base < end; base += 64) {
+      const long long idx = base + lane;
+      if (idx < end) {
+        if (MIXED) s_e[lane] = list32[idx];
+        else {
+          s_d[0][lane] = list64[3 * idx];
+          s_d[1][lane] = list64[3 * idx + 1];
+          s_d[2][lane] = list64[3 * idx + 2];
+        }
+      }
+      wave_sync();
+      const int n = (int)(end - base < 64 ? end - base : 64);
+      if (MIXED) {
+        float aE = 0.f, aG0 = 0.f, aG1 = 0.f, aRc0 = 0.f, aRc1 = 0.f, aRc2 = 0.f, aRu0 = 0.f,
+              aRu1 = 0.f, aRu2 = 0.f;
+        for (int q = 0; q < n; ++q) {
+          const float4 e = s_e[q];
+          const float rx = e.x - cxf, ry = e.y - cyf, rz = e.z - czf;
+          const float rr = rx * rx + ry * ry + rz * rz;
+          const float tt = rx * uxf + ry * uyf + rz * uzf;
+          const bool near = rr < near2;
+          const float d0r = fmaf(tt, m2o0, rr + o0sq);
+          const float d1r = fmaf(tt, m2o1, rr + o1sq);
+          // a near lane contributes exactly -shift here (clamped to rc^2) and the
+          // FP64 reference term below
+          const float d0 = near ? rc2f : __builtin_amdgcn_fmed3f(d0r, fl2f, rc2f);
+          const float d1 = near ? rc2f : __builtin_amdgcn_fmed3f(d1r, fl2f, rc2f);
+          const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1);
+          const float i03 = i0 * i0 * i0, i13 = i1 * i1 * i1;
+          aE = fmaf(i03, fmaf(i03, s6, -1.0f), aE);
+          aE = fmaf(i13, fmaf(i13, s6, -1.0f), aE);
+          if (GRAD) {
+            const bool in0 = !near && d0r > fl2f && d0r < rc2f;
+            const bool in1 = !near && d1r > fl2f && d1r < rc2f;
+            const float g0 = in0 ? i03 * i0 * fmaf(-2.0f * s6, i03, 1.0f) : 0.0f;
+            const float g1 = in1 ? i13 * i1 * fmaf(-2.0f * s6, i13, 1.0f) : 0.0f;
+            aG0 += g0;
+            aG1 += g1;
+            const float w = g0 + g1, v = fmaf(o0f, g0, o1f * g1);
+            aRc0 = fmaf(w, rx, aRc0); aRc1 = fmaf(w, ry, aRc1); aRc2 = fmaf(w, rz, aRc2);
+            aRu0 = fmaf(v, rx, aRu0); aRu1 = fmaf(v, ry, aRu1); aRu2 = fmaf(v, rz, aRu2);
+          }
+          if (near) {
+            const long long id = base + q;
+            const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
+                         qz = list64[3 * id + 2] - cz;
+            pose_pair64(o0, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
+            pose_pair64(o1, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
+          }
+        }
+        a.E = fma(c6, (double)aE, a.E);
+        if (GRAD) {
+          const double k = 3.0 * c6;
+          a.G0 = fma(k, (double)aG0, a.G0);
+          a.G1 = fma(k, (double)aG1, a.G1);
+          a.Rc[0] = fma(k, (double)aRc0, a.Rc[0]);
+          a.Rc[1] = fma(k, (double)aRc1, a.Rc[1]);
+          a.Rc[2] = fma(k, (double)aRc2, a.Rc[2]);
+          a.Ru[0] = fma(k, (double)aRu0, a.Ru[0]);
+          a.Ru[1] = fma(k, (double)aRu1, a.Ru[1]);
+          a.Ru[2] = fma(k, (double)aRu2, a.Ru[2]);
+        }
+      } else {
+        for (int q = 0; q < n; ++q) {
+          const double rx = s_d[0][q] - cx, ry = s_d[1][q] - cy, rz = s_d[2][q] - cz;
+          pose_pair64(o0, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2, GRAD, a);
+          pose_pair64(o1, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2, GRAD, a);
+        }
+      }
+      wave_sync();
+    }
+    // every list entry went through the clamped FP32 term of both sites
+    if (MIXED) a.E = fma((double)(2 * (end - beg)), sh, a.E);
+  }
+  if (!ok) return;
+  const int cid = J.center_id[slot];
+  if (!isfinite(a.E)) atomicOr(J.bad, 1);
+  if (J.pose_e) J.pose_e[cid] = a.E;
+  if (GRAD) {
+    const double go = J.grad_out ? J.grad_out[cid] : 1.0;
+    const double sc = o0 * a.G0 + o1 * a.G1, su = o0 * o0 * a.G0 + o1 * o1 * a.G1;
+    const double u[3] = {ux, uy, uz};
+    for (int k = 0; k < 3; ++k) {
+      const double gc = 2.0 * (u[k] * sc - a.Rc[k] + a.Gc[k]);
+      const double gu = 2.0 * (u[k] * su - a.Ru[k] + a.Gu[k]);
+      if (J.grad_c) J.grad_c[3 * (long long)cid + k] = go * gc;
+      if (J.grad_u) J.grad_u[3 * (long long)cid + k] = go * gu;
+      if (!isfinite(gc) || !isfinite(gu)) atomicOr(J.bad, 1);
+    }
+  }
+}
+
 // Exact count of in-cutoff (site, image) pairs: the pair-eval census that the
 // throughput metric is quoted in.  Same traversal as field_fp64_kernel.
 __global__ __launch_bounds__(256) void census_kernel(const Job* __restrict__ jobs, int n_jobs,
@@ -716,6 +872,25 @@ hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job
   if (n_total <= 0) return hipSuccess;
   hipLaunchKernelGGL(marginal_kernel, dim3((unsigned)n_total), dim3(64), 0, s, d_jobs, n_jobs,
                      d_job_first, n_total);
+  return hipGetLastError();
+}
+
+hipError_t launch_pose(int precision, bool grad, const Job* d_jobs, const SiteChunk* d_chunks,
+                       int n_chunks, const long long* d_offsets, const float4* d_list32,
+                       const double* d_list64, hipStream_t s) {
+  if (n_chunks <= 0) return hipSuccess;
+  const dim3 g(n_chunks), b(64);
+  if (precision == PREC_MIXED) {
+    if (grad)
+      hipLaunchKernelGGL((pose_kernel<true, true>), g, b, 0, s, d_jobs, d_chunks, d_offsets, d_list32, d_list64);
+    else
+      hipLaunchKernelGGL((pose_kernel<true, false>), g, b, 0, s, d_jobs, d_chunks, d_offsets, d_list32, d_list64);
+  } else {
+    if (grad)
+      hipLaunchKernelGGL((pose_kernel<false, true>), g, b, 0, s, d_jobs, d_chunks, d_offsets, d_list32, d_list64);
+    else
+      hipLaunchKernelGGL((pose_kernel<false, false>), g, b, 0, s, d_jobs, d_chunks, d_offsets, d_list32, d_list64);
+  }
   return hipGetLastError();
 }
 

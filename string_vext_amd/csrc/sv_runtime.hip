@@ -125,6 +125,13 @@ struct Poses {
   DevBuf<int> d_center_id, d_tile_of;
   DevBuf<double> w_out64;
   DevBuf<float> w_out32, w_marg, w_grid, w_site;
+  // independent poses: one director per center (sorted slot order)
+  bool per_pose = false;
+  std::vector<double> pose_axes;
+  DevBuf<double> d_pose_axes64;
+  DevBuf<float> d_pose_axes32;
+  struct Batch* pose_batch = nullptr;
+  int pose_precision = -1;
 };
 
 struct Batch {
@@ -148,6 +155,7 @@ struct Batch {
   DevBuf<unsigned long long> d_pairs;
   hipEvent_t ev[12] = {};
   bool timed_last = false;
+  bool lists_built = false;
   ~Batch() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -333,6 +341,8 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
     J.tile_of = S->d_tile_of.p;
     J.axes64 = S->d_axes64.p;
     J.axes32 = S->d_axes32.p;
+    J.pose_axes64 = S->d_pose_axes64.p;
+    J.pose_axes32 = S->d_pose_axes32.p;
     J.n_centers = S->n_centers;
     J.n_axes = S->n_axes;
     J.n_groups = S->n_groups;
@@ -367,7 +377,7 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
       T.job = j;
       B->tiles.push_back(T);
       B->seg_base.push_back(segs + (long long)t * P->n_types);
-      if (J.site32)
+      if (J.site32 || S->per_pose)
         for (int first = 0; first < T.n_members; first += 64)
           B->chunks.push_back({j, (int)t, T.first_member + first, std::min(64, T.n_members - first)});
     }
@@ -443,6 +453,7 @@ int batch_lists(Batch* B, bool timed) {
   SV_HIP(launch_tile_fill(B->d_jobs.p, B->d_tiles.p, nt, B->d_seg_base.p, B->d_offsets.p,
                           B->d_list32.p, B->d_list64.p, B->stream));
   if (timed) SV_HIP(hipEventRecord(B->ev[4], B->stream));
+  B->lists_built = true;
   return 0;
 }
 
@@ -736,7 +747,9 @@ int sv_poses_grid(void* plan, int size, int n_axes, const double* axes, void** p
 }
 
 int sv_poses_destroy(void* poses) {
-  delete static_cast<Poses*>(poses);
+  Poses* S = static_cast<Poses*>(poses);
+  if (S) delete S->pose_batch;
+  delete S;
   return 0;
 }
 
@@ -868,6 +881,90 @@ int sv_field_census(void* plan, void* poses, unsigned long long* pairs) {
   rc = sv_batch_census(B, pairs);
   delete B;
   return rc;
+}
+
+int sv_pose_list_create(void* plan, int n_poses, const double* centers_cart, const double* axes,
+                        void** poses) {
+  if (!plan || !poses || (n_poses > 0 && (!centers_cart || !axes))) return fail(1, "null argument");
+  if (n_poses < 1) return fail(1, "at least one pose required");
+  Plan* P = static_cast<Plan*>(plan);
+  std::vector<double> frac(3 * (size_t)n_poses);
+  for (int i = 0; i < n_poses; ++i) {
+    const double* c = centers_cart + 3 * (size_t)i;
+    const double* u = axes + 3 * (size_t)i;
+    const double nn = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    if (!(std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2])))
+      return fail(1, "nonfinite pose center");
+    if (!std::isfinite(nn) || nn <= 0.0) return fail(1, "finite nonzero directors required");
+    for (int k = 0; k < 3; ++k)
+      frac[3 * (size_t)i + k] = c[0] * P->Hinv[k] + c[1] * P->Hinv[3 + k] + c[2] * P->Hinv[6 + k];
+  }
+  const double unit[3] = {1.0, 0.0, 0.0};
+  Poses* S = nullptr;
+  int rc = build_poses(P, n_poses, frac.data(), 1, unit, 0, &S);
+  if (rc) return rc;
+  S->per_pose = true;
+  S->pose_axes.resize(3 * (size_t)n_poses);
+  std::vector<float> a32(3 * (size_t)n_poses);
+  for (int slot = 0; slot < n_poses; ++slot)
+    for (int k = 0; k < 3; ++k) {
+      const double v = axes[3 * (size_t)S->center_id[slot] + k];
+      S->pose_axes[3 * (size_t)slot + k] = v;
+      a32[3 * (size_t)slot + k] = (float)v;
+    }
+  hipError_t e;
+  if ((e = S->d_pose_axes64.upload(S->pose_axes.data(), S->pose_axes.size(), P->stream)) != hipSuccess ||
+      (e = S->d_pose_axes32.upload(a32.data(), a32.size(), P->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
+    delete S;
+    return hip_fail(e, "pose upload");
+  }
+  *poses = S;
+  return 0;
+}
+
+int sv_pose_list_eval(void* poses, int precision, const double* grad_out, double* energy,
+                      double* grad_center, double* grad_axis) {
+  if (!poses) return fail(1, "null pose list");
+  Poses* S = static_cast<Poses*>(poses);
+  if (!S->per_pose) return fail(1, "pose set was not created by sv_pose_list_create");
+  if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
+  Plan* P = S->plan;
+  SV_HIP(hipSetDevice(P->device));
+  if (S->pose_batch && S->pose_precision != precision) {
+    delete S->pose_batch;
+    S->pose_batch = nullptr;
+  }
+  if (!S->pose_batch) {
+    Batch* B = nullptr;
+    Plan* pl = P;
+    Poses* po = S;
+    int rc = batch_build(1, &pl, &po, precision, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, &B);
+    if (rc) return rc;
+    S->pose_batch = B;
+    S->pose_precision = precision;
+  }
+  Batch* B = S->pose_batch;
+  Job& J = B->jobs[0];
+  J.grad_out = grad_out;
+  J.pose_e = energy;
+  J.grad_c = grad_center;
+  J.grad_u = grad_axis;
+  SV_HIP(hipMemcpyAsync(B->d_jobs.p, &J, sizeof(Job), hipMemcpyHostToDevice, B->stream));
+  if (!B->lists_built) {
+    int rc = batch_lists(B, false);
+    if (rc) return rc;
+  }
+  SV_HIP(launch_zero(B->d_bad.p, 1, B->stream));
+  const bool grad = grad_center || grad_axis;
+  SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)B->chunks.size(),
+                     B->d_offsets.p, B->d_list32.p, B->d_list64.p, B->stream));
+  int bad = 0;
+  SV_HIP(hipMemcpyAsync(&bad, B->d_bad.p, sizeof(int), hipMemcpyDeviceToHost, B->stream));
+  SV_HIP(hipStreamSynchronize(B->stream));
+  if (bad) return fail(4, "nonfinite pose energy or gradient");
+  return 0;
 }
 
 int sv_device_malloc(void** ptr, uint64_t bytes) {

@@ -260,3 +260,49 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+class PoseList:
+    """Independent poses (center_i, director_i) on the device (sv_pose_list_create).
+
+    ``centers_cart`` are Cartesian (Angstrom); directors are used as given.
+    ``eval`` writes device arrays: pass raw device pointers (e.g. torch
+    ``data_ptr()``) or leave them ``None`` to skip an output.
+    """
+
+    def __init__(self, plan: Plan, centers_cart, axes):
+        self.lib = plan.lib
+        self.plan = plan
+        centers = _f64(centers_cart)
+        axes = _f64(axes)
+        if centers.ndim != 2 or centers.shape[1] != 3 or axes.shape != centers.shape:
+            raise ValueError("centers and axes must both be (n, 3)")
+        handle = ctypes.c_void_p()
+        _lib.check(self.lib.sv_pose_list_create(plan.handle, len(centers), _ptr(centers),
+                                                _ptr(axes), ctypes.byref(handle)))
+        self.handle = handle.value
+        self.n = len(centers)
+
+    def eval(self, precision="fp64", grad_out=None, energy=None, grad_center=None, grad_axis=None):
+        """Device pointers (ints) for every array; returns nothing."""
+        _lib.check(self.lib.sv_pose_list_eval(self.handle, precision_code(precision), grad_out,
+                                              energy, grad_center, grad_axis))
+
+    def energy_host(self, precision="fp64", grad=False):
+        """Convenience: energies (and gradients) copied back to numpy."""
+        bufs = [DeviceBuffer(8 * self.n)]
+        if grad:
+            bufs += [DeviceBuffer(24 * self.n), DeviceBuffer(24 * self.n)]
+        self.eval(precision, None, bufs[0].ptr, *(b.ptr for b in bufs[1:]))
+        out = [bufs[0].to_host(np.float64, (self.n,))]
+        if grad:
+            out += [b.to_host(np.float64, (self.n, 3)) for b in bufs[1:]]
+        return out if grad else out[0]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.sv_poses_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,170 @@
+"""GPU parity of independent-pose energies and their hand-written gradient
+(sv_pose_list_*, string_vext_amd.autograd) against the FP64 oracle, central
+finite differences, and the String solver's forward-difference forces."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def case():
+    from string_vext_amd import synthetic
+    return synthetic.small_field_case()
+
+
+@pytest.fixture(scope="module")
+def poses(case):
+    rng = np.random.default_rng(20261104)
+    n = 700
+    frac = rng.random((n, 3))
+    frac[:8] = [[.001, .999, .5], [.999, .001, .5], [.5, .001, .999], [.001, .5, .999],
+                [.999, .5, .001], [.5, .999, .001], [.001, .001, .001], [.999, .999, .999]]
+    axes = rng.normal(size=(n, 3))
+    axes /= np.linalg.norm(axes, axis=1, keepdims=True)
+    return frac @ np.asarray(case["cell"]), axes
+
+
+def _plan(case):
+    from string_vext_amd.field import Plan
+    return Plan.from_case(case)
+
+
+def test_pose_energy_and_gradient_fp64_match_oracle(oracle, case, poses):
+    from string_vext_amd.field import PoseList
+    centers, axes = poses
+    e_ref, gc_ref, gu_ref = oracle.pose_grad(case, centers, axes)
+    pl = PoseList(_plan(case), centers, axes)
+    e, gc, gu = pl.energy_host("fp64", grad=True)
+    scale = np.maximum(np.abs(e_ref), 1.0)
+    assert (np.abs(e - e_ref) / scale).max() < 1e-12
+    gscale = np.maximum(np.abs(gc_ref).max(axis=1, keepdims=True), 1.0)
+    assert (np.abs(gc - gc_ref) / gscale).max() < 1e-10
+    gscale = np.maximum(np.abs(gu_ref).max(axis=1, keepdims=True), 1.0)
+    assert (np.abs(gu - gu_ref) / gscale).max() < 1e-10
+
+
+def test_pose_energy_and_gradient_mixed_within_tolerance(oracle, case, poses):
+    from string_vext_amd.field import PoseList
+    centers, axes = poses
+    e_ref, gc_ref, gu_ref = oracle.pose_grad(case, centers, axes)
+    e, gc, gu = PoseList(_plan(case), centers, axes).energy_host("mixed", grad=True)
+    m = oracle.error_metrics(e_ref, e)
+    assert m["accessible_max_abs_error_K"] < 1e-3
+    assert m["all_max_relative_error_floor1K"] < 1e-5
+    # gradient: K/A; relative to the pose's gradient scale with a 1 K/A floor
+    for g, ref in ((gc, gc_ref), (gu, gu_ref)):
+        scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)
+        assert (np.abs(g - ref) / scale).max() < 1e-4
+
+
+def test_gradient_matches_central_differences(case, poses):
+    """h = 1e-4 A on the GPU FP64 energies (config 4's check)."""
+    from string_vext_amd.field import PoseList
+    centers, axes = poses
+    centers, axes = centers[:120], axes[:120]
+    plan = _plan(case)
+    e, gc, gu = PoseList(plan, centers, axes).energy_host("fp64", grad=True)
+    h = 1e-4
+    for k in range(3):
+        step = np.zeros(3)
+        step[k] = h
+        ep = PoseList(plan, centers + step, axes).energy_host("fp64")
+        em = PoseList(plan, centers - step, axes).energy_host("fp64")
+        fd = (ep - em) / (2 * h)
+        ok = np.abs(e) < 1e4    # central differences of overlap poses are not informative
+        np.testing.assert_allclose(gc[ok, k], fd[ok], rtol=1e-5, atol=1e-4)
+        ep = PoseList(plan, centers, axes + step).energy_host("fp64")
+        em = PoseList(plan, centers, axes - step).energy_host("fp64")
+        fd = (ep - em) / (2 * h)
+        np.testing.assert_allclose(gu[ok, k], fd[ok], rtol=1e-5, atol=1e-4)
+
+
+def test_string_solver_energy_and_forward_difference_forces(oracle, case):
+    """The String solver's pose energy (triclinic_pair.h summed over sites and
+    atoms, body-x guest rotated by Rz(g)Ry(b)Rx(a), device_fxn.h:34-63) and its
+    unnormalized forward-difference forces g_k = V(x + d e_k) - V(x) with the
+    input's deltas (global.h:1281-1324; d_frac = 1e-4, d_angle = 1 deg) against
+    the analytic gradient by the chain rule."""
+    from string_vext_amd.field import PoseList
+    cell = np.asarray(case["cell"])
+    off = oracle.body_offsets(case)
+    gs, ge = case["guest_sigma"][0], case["guest_epsilon"][0]
+    rng = np.random.default_rng(9)
+
+    def director(beta, gamma):
+        return np.array([np.cos(gamma) * np.cos(beta), np.sin(gamma) * np.cos(beta), -np.sin(beta)])
+
+    def v_string(x):
+        c = x[:3] @ cell
+        u = director(x[4], x[5])
+        total = 0.0
+        for o in off:
+            site = c + o * u
+            for f, s, e in zip(case["frame_frac"], case["frame_sigma"], case["frame_epsilon"]):
+                total += oracle.string_pair(site, f, cell, ge, gs, e, s, case["cutoff"])
+        return total
+
+    plan = _plan(case)
+    # accessible poses (E < 2000 K) found by a GPU scan of random poses
+    xs_all = np.c_[rng.random((4000, 3)), rng.uniform(0, 2 * np.pi, (4000, 3))]
+    dirs = np.array([director(b, g) for b, g in xs_all[:, 4:6]])
+    scan = PoseList(plan, xs_all[:, :3] @ cell, dirs).energy_host("fp64")
+    picks = xs_all[(scan < 2000.0) & (scan > -1e4)][:6]
+    assert len(picks) >= 3
+    tried = 0
+    for x in picks:
+        c, u = x[:3] @ cell, director(x[4], x[5])
+        e, gc, gu = PoseList(plan, c[None], u[None]).energy_host("fp64", grad=True)
+        v = v_string(x)
+        tried += 1
+        assert e[0] == pytest.approx(v, rel=1e-10, abs=1e-7)
+        deltas = [1e-4, 1e-4, 1e-4, np.radians(1.0), np.radians(1.0), np.radians(1.0)]
+        for k in (0, 1, 2, 4, 5):
+            xs = x.copy()
+            xs[k] += deltas[k]
+            g_fd = v_string(xs) - v
+            # analytic derivative at the midpoint: forward difference is second-order there
+            xm = x.copy()
+            xm[k] += deltas[k] / 2
+            cm, um = xm[:3] @ cell, director(xm[4], xm[5])
+            _, gcm, gum = PoseList(plan, cm[None], um[None]).energy_host("fp64", grad=True)
+            if k < 3:
+                deriv = (cell @ gcm[0])[k]
+            elif k == 4:
+                deriv = gum[0] @ np.array([-np.cos(xm[5]) * np.sin(xm[4]),
+                                           -np.sin(xm[5]) * np.sin(xm[4]), -np.cos(xm[4])])
+            else:
+                deriv = gum[0] @ np.array([-np.sin(xm[5]) * np.cos(xm[4]),
+                                           np.cos(xm[5]) * np.cos(xm[4]), 0.0])
+            # the midpoint-rule residual of a forward difference is f''' d^3 / 24:
+            # ~1e-5 relative for d = 1e-4 fractional (2.5e-3 A), ~1e-3 for 1 deg
+            tol = 1e-4 if k < 3 else 5e-3
+            assert g_fd == pytest.approx(deriv * deltas[k], rel=tol, abs=1e-6)
+    assert tried >= 3
+
+
+def test_autograd_function(oracle, case, poses):
+    import torch
+    from string_vext_amd.autograd import pose_energy
+    centers, axes = poses
+    e_ref, gc_ref, gu_ref = oracle.pose_grad(case, centers, axes)
+    plan = _plan(case)
+    c = torch.tensor(centers, dtype=torch.float64, device="cuda", requires_grad=True)
+    u = torch.tensor(axes, dtype=torch.float64, device="cuda", requires_grad=True)
+    w = torch.linspace(0.5, 1.5, len(centers), dtype=torch.float64, device="cuda")
+    e = pose_energy(plan, c, u, "fp64")
+    (e * w).sum().backward()
+    wn = w.cpu().numpy()[:, None]
+    np.testing.assert_allclose(e.detach().cpu().numpy(), e_ref, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(c.grad.cpu().numpy(), wn * gc_ref, rtol=1e-9,
+                               atol=1e-9 * np.abs(gc_ref).max())
+    np.testing.assert_allclose(u.grad.cpu().numpy(), wn * gu_ref, rtol=1e-9,
+                               atol=1e-9 * np.abs(gu_ref).max())
+    # normalized directors as free parameters: autograd carries the normalization
+    v = torch.tensor(axes * 2.5, dtype=torch.float64, device="cuda", requires_grad=True)
+    e2 = pose_energy(plan, c.detach(), v / v.norm(dim=1, keepdim=True), "mixed")
+    e2.sum().backward()
+    assert torch.isfinite(v.grad).all()
+    radial = (v.grad * v.detach()).sum(dim=1).abs().max().item()
+    assert radial < 1e-6 * max(1.0, v.grad.abs().max().item())
