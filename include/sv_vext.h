@@ -147,6 +147,34 @@ int sv_pose_list_create(void *plan, int n_poses, const double *centers_cart, con
 int sv_pose_list_eval(void *poses, int precision, const double *grad_out, double *energy,
                       double *grad_center, double *grad_axis);
 
+/* ------------------------------------------------------------------------ */
+/* Batched String method: n_strings independent strings of n_images images. */
+/* Image state x = (a, b, c, alpha, beta, gamma): fractional center and      */
+/* Euler angles (rad) of the body-x guest, director (cos g cos b,            */
+/* sin g cos b, -sin b) (native/string/source_code/device_fxn.h:34-63).      */
+/* Each iteration follows the reference iteration (RC_main.cu:1914-2393):    */
+/* normalized steepest descent with move_frac / move_angle_rad and the       */
+/* analytic gradient in place of forward differences, endpoint fix along     */
+/* `direction` (1 = a, 2 = b, 3 = c), arc-length reparametrization, angle    */
+/* revert and smoothing, convergence test every 200 iterations (a converged  */
+/* string stops).  plans[s] is the framework plan of string s.               */
+/*   init [n_strings][n_images][6]; n_images in 3..1024.                     */
+/* sv_strings_run runs `iterations` more iterations; with use_graph one       */
+/* iteration is captured once in a HIP graph and replayed.                   */
+/* sv_strings_state returns the current strings (host arrays, may be NULL):  */
+/*   path [S][N][6], energy [S][N] (evaluated now), iterations done [S],     */
+/*   converged [S].                                                           */
+/* No counterpart in the reference's C interface (its String solver is a     */
+/* CUDA executable, src/string_vext/string.py:33-39).                         */
+/* ------------------------------------------------------------------------ */
+int sv_strings_create(int n_strings, void *const *plans, int n_images, const double *init,
+                      const int *direction, double move_frac, double move_angle_rad,
+                      int precision, void **strings);
+int sv_strings_run(void *strings, int iterations, int use_graph);
+int sv_strings_state(void *strings, double *path, double *energy, int *iterations,
+                     int *converged);
+int sv_strings_destroy(void *strings);
+
 /* Device memory helpers for callers without their own allocator. */
 int sv_device_malloc(void **ptr, uint64_t bytes);
 int sv_device_free(void *ptr);

@@ -40,6 +40,10 @@ SIGNATURES = [
     ("sv_batch_destroy", _I, [_P]),
     ("sv_pose_list_create", _I, [_P, _I, _P, _P, _P]),
     ("sv_pose_list_eval", _I, [_P, _I, _P, _P, _P, _P]),
+    ("sv_strings_create", _I, [_I, _P, _I, _P, _P, _D, _D, _I, _P]),
+    ("sv_strings_run", _I, [_P, _I, _I]),
+    ("sv_strings_state", _I, [_P, _P, _P, _P, _P]),
+    ("sv_strings_destroy", _I, [_P]),
     ("sv_device_malloc", _I, [_P, ctypes.c_uint64]),
     ("sv_device_free", _I, [_P]),
     ("sv_memcpy_d2h", _I, [_P, _P, ctypes.c_uint64]),

@@ -1,0 +1,118 @@
+// Host-side objects of libsv_vext.so shared by the runtime translation units.
+#pragma once
+#include "sv_internal.h"
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace sv {
+
+// Device allocation owned by one host object (grown, never shrunk).
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  hipError_t reserve(size_t n, bool grow = false) {
+    if (n <= cap && p) return hipSuccess;
+    release();
+    size_t want = std::max<size_t>(n, 1);
+    if (grow) want += want / 4;
+    hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    else p = nullptr;
+    return e;
+  }
+  hipError_t upload(const T* h, size_t n, hipStream_t s) {
+    hipError_t e = reserve(n);
+    if (e != hipSuccess || n == 0) return e;
+    return hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+struct Plan {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double H[9], Hinv[9], hn[3], volume;
+  int n_atoms = 0, n_types = 0;
+  std::vector<double> frac;       // sorted by type, wrapped
+  std::vector<int> type_start;
+  std::vector<double> c12, c6, shift, floor2, sigma, eps;
+  std::vector<float> s6f, floor2f, near2f, lo2f;
+  double o0, o1, cutoff, keep_r, near_energy;
+  DevBuf<double> d_frac, d_c12, d_c6, d_shift, d_floor2, d_sigma, d_eps;
+  DevBuf<int> d_type_start;
+  DevBuf<float> d_s6f, d_floor2f, d_near2f, d_lo2f;
+  ~Plan() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct Poses {
+  Plan* plan = nullptr;
+  int n_centers = 0, n_axes = 0, n_groups = 0;
+  std::vector<double> axes;
+  std::vector<Tile> tiles;        // job field left 0; filled per batch
+  std::vector<int> center_id, tile_of;
+  std::vector<double> center_rel;
+  DevBuf<double> d_center_rel, d_axes64;
+  DevBuf<float> d_axes32;
+  DevBuf<int> d_center_id, d_tile_of;
+  DevBuf<double> w_out64;
+  DevBuf<float> w_out32, w_marg, w_grid, w_site;
+  // independent poses: one director per center (sorted slot order)
+  bool per_pose = false;
+  std::vector<double> pose_axes;
+  DevBuf<double> d_pose_axes64;
+  DevBuf<float> d_pose_axes32;
+  struct Batch* pose_batch = nullptr;
+  int pose_precision = -1;
+};
+
+struct Batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_jobs = 0, precision = 0;
+  bool all_sym = true, need_marginal = false, any_site = false;
+  std::vector<Job> jobs;
+  std::vector<long long> job_work, job_first;
+  std::vector<Tile> tiles;
+  std::vector<long long> seg_base;
+  std::vector<SiteChunk> chunks;
+  long long n_segs = 0, n_work = 0, n_centers_total = 0, list_total = -1;
+  DevBuf<Job> d_jobs;
+  DevBuf<long long> d_job_work, d_job_first, d_seg_base, d_offsets;
+  DevBuf<Tile> d_tiles;
+  DevBuf<SiteChunk> d_chunks;
+  DevBuf<int> d_counts, d_bad;
+  DevBuf<float4> d_list32;
+  DevBuf<double> d_list64;
+  DevBuf<unsigned long long> d_pairs;
+  hipEvent_t ev[12] = {};
+  bool timed_last = false;
+  bool lists_built = false;
+  ~Batch() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+
+// Error reporting (thread-local message, C-ABI return code).
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// Framework and guest fields of a job from a plan (outputs and pose fields untouched).
+void plan_fill_job(const Plan& P, Job& J);
+
+}  // namespace sv

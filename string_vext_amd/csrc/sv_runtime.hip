@@ -1,6 +1,6 @@
 // Host runtime of libsv_vext.so: FP64 geometry setup, pose tiling, device
 // tables, launch orchestration and the C ABI declared in include/sv_vext.h.
-#include "sv_internal.h"
+#include "sv_objects.h"
 #include "sv_vext.h"
 
 #include <algorithm>
@@ -17,8 +17,6 @@ namespace sv {
 hipError_t launch_zero(int* p, int n, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 
-namespace {
-
 thread_local std::string t_error;
 
 int fail(int code, const std::string& msg) {
@@ -30,41 +28,13 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? 6 : 5, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+namespace {
+
 #define SV_HIP(call)                                   \
   do {                                                 \
     hipError_t sv_e_ = (call);                         \
     if (sv_e_ != hipSuccess) return hip_fail(sv_e_, #call); \
   } while (0)
-
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t cap = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-  hipError_t reserve(size_t n, bool grow = false) {
-    if (n <= cap && p) return hipSuccess;
-    release();
-    size_t want = std::max<size_t>(n, 1);
-    if (grow) want += want / 4;
-    hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
-    if (e == hipSuccess) cap = want;
-    else p = nullptr;
-    return e;
-  }
-  hipError_t upload(const T* h, size_t n, hipStream_t s) {
-    hipError_t e = reserve(n);
-    if (e != hipSuccess || n == 0) return e;
-    return hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
-  }
-};
 
 // numpy's np.mod(x, 1.0) for finite x (result in [0, 1], 1.0 only for tiny
 // negative x), used by the reference to wrap fractional coordinates.
@@ -95,73 +65,37 @@ bool invert3(const double* H, double* inv, double* det_out) {
 
 }  // namespace
 
-struct Plan {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  double H[9], Hinv[9], hn[3], volume;
-  int n_atoms = 0, n_types = 0;
-  std::vector<double> frac;       // sorted by type, wrapped
-  std::vector<int> type_start;
-  std::vector<double> c12, c6, shift, floor2, sigma, eps;
-  std::vector<float> s6f, floor2f, near2f, lo2f;
-  double o0, o1, cutoff, keep_r, near_energy;
-  DevBuf<double> d_frac, d_c12, d_c6, d_shift, d_floor2, d_sigma, d_eps;
-  DevBuf<int> d_type_start;
-  DevBuf<float> d_s6f, d_floor2f, d_near2f, d_lo2f;
-  ~Plan() {
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
-
-struct Poses {
-  Plan* plan = nullptr;
-  int n_centers = 0, n_axes = 0, n_groups = 0;
-  std::vector<double> axes;
-  std::vector<Tile> tiles;        // job field left 0; filled per batch
-  std::vector<int> center_id, tile_of;
-  std::vector<double> center_rel;
-  DevBuf<double> d_center_rel, d_axes64;
-  DevBuf<float> d_axes32;
-  DevBuf<int> d_center_id, d_tile_of;
-  DevBuf<double> w_out64;
-  DevBuf<float> w_out32, w_marg, w_grid, w_site;
-  // independent poses: one director per center (sorted slot order)
-  bool per_pose = false;
-  std::vector<double> pose_axes;
-  DevBuf<double> d_pose_axes64;
-  DevBuf<float> d_pose_axes32;
-  struct Batch* pose_batch = nullptr;
-  int pose_precision = -1;
-};
-
-struct Batch {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int n_jobs = 0, precision = 0;
-  bool all_sym = true, need_marginal = false, any_site = false;
-  std::vector<Job> jobs;
-  std::vector<long long> job_work, job_first;
-  std::vector<Tile> tiles;
-  std::vector<long long> seg_base;
-  std::vector<SiteChunk> chunks;
-  long long n_segs = 0, n_work = 0, n_centers_total = 0, list_total = -1;
-  DevBuf<Job> d_jobs;
-  DevBuf<long long> d_job_work, d_job_first, d_seg_base, d_offsets;
-  DevBuf<Tile> d_tiles;
-  DevBuf<SiteChunk> d_chunks;
-  DevBuf<int> d_counts, d_bad;
-  DevBuf<float4> d_list32;
-  DevBuf<double> d_list64;
-  DevBuf<unsigned long long> d_pairs;
-  hipEvent_t ev[12] = {};
-  bool timed_last = false;
-  bool lists_built = false;
-  ~Batch() {
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
+void plan_fill_job(const Plan& P, Job& J) {
+  std::memcpy(J.H, P.H, sizeof(J.H));
+  std::memcpy(J.hn, P.hn, sizeof(J.hn));
+  J.frac = P.d_frac.p;
+  J.type_start = P.d_type_start.p;
+  J.n_atoms = P.n_atoms;
+  J.n_types = P.n_types;
+  J.o0 = P.o0;
+  J.o1 = P.o1;
+  J.rc2 = P.cutoff * P.cutoff;
+  J.keep_r = P.keep_r;
+  J.rc2f = (float)J.rc2;
+  J.keep2f = (float)(P.keep_r * P.keep_r);
+  J.m2o0f = (float)(-2.0 * P.o0);
+  J.m2o1f = (float)(-2.0 * P.o1);
+  J.o0sqf = (float)(P.o0 * P.o0);
+  J.dR2f = (float)(P.o1 * P.o1 - P.o0 * P.o0);
+  const double in_r = P.cutoff - std::max(std::fabs(P.o0), std::fabs(P.o1)) - 1e-3;
+  J.in2f = in_r > 0.0 ? (float)(in_r * in_r) : 0.0f;
+  J.sym = (P.o1 == -P.o0);
+  J.c12 = P.d_c12.p;
+  J.c6 = P.d_c6.p;
+  J.shift = P.d_shift.p;
+  J.floor2 = P.d_floor2.p;
+  J.sigma = P.d_sigma.p;
+  J.eps = P.d_eps.p;
+  J.s6f = P.d_s6f.p;
+  J.floor2f = P.d_floor2f.p;
+  J.near2f = P.d_near2f.p;
+  J.lo2f = P.d_lo2f.p;
+}
 
 namespace {
 
@@ -304,38 +238,8 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
     }
     Job& J = B->jobs[j];
     std::memset(&J, 0, sizeof(Job));
-    std::memcpy(J.H, P->H, sizeof(J.H));
-    std::memcpy(J.hn, P->hn, sizeof(J.hn));
-    J.frac = P->d_frac.p;
-    J.type_start = P->d_type_start.p;
-    J.n_atoms = P->n_atoms;
-    J.n_types = P->n_types;
-    J.o0 = P->o0;
-    J.o1 = P->o1;
-    J.rc2 = P->cutoff * P->cutoff;
-    J.keep_r = P->keep_r;
-    J.rc2f = (float)J.rc2;
-    J.keep2f = (float)(P->keep_r * P->keep_r);
-    J.m2o0f = (float)(-2.0 * P->o0);
-    J.m2o1f = (float)(-2.0 * P->o1);
-    J.o0sqf = (float)(P->o0 * P->o0);
-    J.dR2f = (float)(P->o1 * P->o1 - P->o0 * P->o0);
-    {
-      const double in_r = P->cutoff - std::max(std::fabs(P->o0), std::fabs(P->o1)) - 1e-3;
-      J.in2f = in_r > 0.0 ? (float)(in_r * in_r) : 0.0f;
-    }
-    J.sym = (P->o1 == -P->o0);
+    plan_fill_job(*P, J);
     if (!J.sym) B->all_sym = false;
-    J.c12 = P->d_c12.p;
-    J.c6 = P->d_c6.p;
-    J.shift = P->d_shift.p;
-    J.floor2 = P->d_floor2.p;
-    J.sigma = P->d_sigma.p;
-    J.eps = P->d_eps.p;
-    J.s6f = P->d_s6f.p;
-    J.floor2f = P->d_floor2f.p;
-    J.near2f = P->d_near2f.p;
-    J.lo2f = P->d_lo2f.p;
     J.center_rel = S->d_center_rel.p;
     J.center_id = S->d_center_id.p;
     J.tile_of = S->d_tile_of.p;

@@ -1,0 +1,516 @@
+// Batched String method on the GPU.
+//
+// Behaviour follows the reference String solver's iteration
+// (native/string/source_code/RC_main.cu:1914-2393, kernels of global.h:
+// 1281-2240), with its forward-difference forces replaced by the analytic
+// gradient of the same pose energy:
+//   1. force of image i: g_k = delta_k dV/dx_k on x = (a, b, c, alpha, beta,
+//      gamma), delta = (move_frac x3, move_angle x3), each floored to a 1e-15
+//      grid (global.h:1281-1324; a forward difference V(x + delta_k) - V(x)
+//      is delta_k dV/dx_k to first order);
+//   2. step with separate translational / rotational norms: s1 = s0 - delta g
+//      when |g| < delta, else s0 - delta g/|g| (global.h:1328-1443);
+//   3. endpoints fixed along the hop direction, the last image a periodic
+//      replica of the first, interior images outside [0, 1] along the hop
+//      direction replaced by the nearest in-range neighbour (global.h:1456-1647;
+//      applied in image order here, the reference does it in parallel);
+//   4. arc-length reparametrization of (a, b, c) by linear interpolation at
+//      equal Cartesian arc-length fractions, angles unchanged (global.h:1653-1846);
+//   5. angles of images whose energy rose are reverted, angles smoothed with
+//      coefficient 1e-4 and the reference's periodic-unwrap branches
+//      (global.h:1854-2085);
+//   6. every 200 iterations the string stops once the summed per-image
+//      coordinate and orientation changes fall below 30% of
+//      move x n_images x sqrt(3) (RC_main.cu:1853-1854, 2361-2376);
+//   7. s0 <- s2 with smoothed angles wrapped into [0, 2 pi] (global.h:2169-2240).
+// The director of the body-x guest is (cos g cos b, sin g cos b, -sin b)
+// (device_fxn.h:34-63), so alpha carries no force.
+//
+// Energies of string images are evaluated one wave per image, lanes over
+// framework atoms and their lattice images inside rc + max|o| (no candidate
+// list: images move every iteration).  One iteration is four kernels; the
+// runtime captures them in a HIP graph and replays it.
+#include "sv_objects.h"
+#include "sv_vext.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace sv {
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kRound = 1e-15;   // RC_main.cu:1733
+constexpr double kSmooth = 1e-4;   // RC_main.cu:1734
+constexpr int kCheckEvery = 200;   // RC_main.cu:2361
+
+struct StringParams {
+  int n_strings;
+  int n_images;
+  double move_frac;
+  double move_angle;
+  double conv_c;   // coordinate threshold
+  double conv_o;   // orientation threshold
+};
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double grid_floor(double v) { return floor(v / kRound) * kRound; }
+
+// Energy (and gradient) of one pose per wave.  x = (a, b, c, alpha, beta,
+// gamma) with fractional center (a, b, c).  G receives dV/dx.
+template <bool MIXED, bool GRAD>
+__global__ __launch_bounds__(256) void image_kernel(const Job* __restrict__ plans,
+                                                    const int* __restrict__ plan_of, int n_images,
+                                                    int n_poses, const int* __restrict__ converged,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ V, double* __restrict__ G) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + uniform_i(threadIdx.x >> 6);
+  if (p >= n_poses) return;
+  const int s = p / n_images;
+  if (converged && converged[s]) return;
+  const Job& J = plans[plan_of[s]];
+  const double f[3] = {x[6 * p], x[6 * p + 1], x[6 * p + 2]};
+  const double be = x[6 * p + 4], ga = x[6 * p + 5];
+  const double u[3] = {cos(ga) * cos(be), sin(ga) * cos(be), -sin(be)};
+  const float uf[3] = {(float)u[0], (float)u[1], (float)u[2]};
+  const double o[2] = {J.o0, J.o1};
+  const double keep_r = J.keep_r, keep2 = keep_r * keep_r, rc2 = J.rc2;
+  double E = 0.0, gc[3] = {0.0, 0.0, 0.0}, gu[3] = {0.0, 0.0, 0.0};
+  for (int t = 0; t < J.n_types; ++t) {
+    const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
+    const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
+    for (int a = J.type_start[t] + lane; a < J.type_start[t + 1]; a += 64) {
+      double d[3];
+      for (int k = 0; k < 3; ++k) d[k] = J.frac[3 * a + k] - f[k];
+      int lo[3], hi[3];
+      for (int k = 0; k < 3; ++k) {
+        const double h = keep_r * J.hn[k];
+        lo[k] = (int)ceil(-d[k] - h);
+        hi[k] = (int)floor(-d[k] + h);
+      }
+      for (int na = lo[0]; na <= hi[0]; ++na)
+        for (int nb = lo[1]; nb <= hi[1]; ++nb)
+          for (int nc = lo[2]; nc <= hi[2]; ++nc) {
+            const double ua = d[0] + na, ub = d[1] + nb, uc = d[2] + nc;
+            const double r[3] = {ua * J.H[0] + ub * J.H[3] + uc * J.H[6],
+                                 ua * J.H[1] + ub * J.H[4] + uc * J.H[7],
+                                 ua * J.H[2] + ub * J.H[5] + uc * J.H[8]};
+            const double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            if (!(rr < keep2)) continue;
+            if (!MIXED || rr < (double)near2) {
+              for (int si = 0; si < 2; ++si) {
+                const double dv[3] = {o[si] * u[0] - r[0], o[si] * u[1] - r[1], o[si] * u[2] - r[2]};
+                const double d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+                if (d2 < rc2) {
+                  const bool floored = d2 < fl2;
+                  const double inv = 1.0 / (floored ? fl2 : d2);
+                  const double i6 = inv * inv * inv;
+                  E += c12 * i6 * i6 - c6 * i6 + sh;
+                  if (GRAD && !floored) {
+                    const double g = 2.0 * (-6.0 * c12 * i6 * i6 + 3.0 * c6 * i6) * inv;
+                    for (int k = 0; k < 3; ++k) {
+                      gc[k] += g * dv[k];
+                      gu[k] += g * o[si] * dv[k];
+                    }
+                  }
+                }
+              }
+            } else {
+              // FP32 pair arithmetic, clamped to [floor2, rc^2] (a clamped
+              // pair gives -shift, compensated by the +shift added per pair)
+              const float rf[3] = {(float)r[0], (float)r[1], (float)r[2]};
+              const float rrf = rf[0] * rf[0] + rf[1] * rf[1] + rf[2] * rf[2];
+              const float tt = rf[0] * uf[0] + rf[1] * uf[1] + rf[2] * uf[2];
+              float e = 0.f;
+              for (int si = 0; si < 2; ++si) {
+                const float of = (float)o[si];
+                const float d2r = fmaf(tt, -2.0f * of, rrf + of * of);
+                const float d2 = __builtin_amdgcn_fmed3f(d2r, fl2f, J.rc2f);
+                const float i = __builtin_amdgcn_rcpf(d2);
+                const float i3 = i * i * i;
+                e = fmaf(i3, fmaf(i3, s6, -1.0f), e);
+                if (GRAD && d2r > fl2f && d2r < J.rc2f) {
+                  const double g = 6.0 * c6 * (double)(i3 * i * fmaf(-2.0f * s6, i3, 1.0f));
+                  for (int k = 0; k < 3; ++k) {
+                    const double dk = o[si] * u[k] - r[k];
+                    gc[k] += g * dk;
+                    gu[k] += g * o[si] * dk;
+                  }
+                }
+              }
+              E += fma(c6, (double)e, 2.0 * sh);
+            }
+          }
+    }
+  }
+  E = wave_sum_d(E);
+  if (GRAD)
+    for (int k = 0; k < 3; ++k) {
+      gc[k] = wave_sum_d(gc[k]);
+      gu[k] = wave_sum_d(gu[k]);
+    }
+  if (lane == 0) {
+    V[p] = E;
+    if (GRAD) {
+      // c = f H: dV/df_k = sum_i H[k][i] dV/dc_i
+      for (int k = 0; k < 3; ++k)
+        G[6 * p + k] = J.H[3 * k] * gc[0] + J.H[3 * k + 1] * gc[1] + J.H[3 * k + 2] * gc[2];
+      G[6 * p + 3] = 0.0;
+      G[6 * p + 4] = gu[0] * (-cos(ga) * sin(be)) + gu[1] * (-sin(ga) * sin(be)) + gu[2] * (-cos(be));
+      G[6 * p + 5] = gu[0] * (-sin(ga) * cos(be)) + gu[1] * (cos(ga) * cos(be));
+    }
+  }
+}
+
+// Steps 1-4: forces -> s1 -> endpoint fix -> reparametrized s2.
+__global__ __launch_bounds__(256) void string_step_kernel(const Job* __restrict__ plans,
+                                                          const int* __restrict__ plan_of,
+                                                          const int* __restrict__ direction,
+                                                          StringParams sp,
+                                                          const int* __restrict__ converged,
+                                                          const double* __restrict__ x0,
+                                                          const double* __restrict__ G,
+                                                          double* __restrict__ x2) {
+  extern __shared__ double sh[];
+  const int s = blockIdx.x;
+  if (converged[s]) return;
+  const int N = sp.n_images;
+  double* s1 = sh;            // [N][6]
+  double* cum = sh + 6 * N;   // [N]
+  const Job& J = plans[plan_of[s]];
+  const int base = s * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int p = base + i;
+    double g[6];
+    for (int k = 0; k < 6; ++k) g[k] = grid_floor((k < 3 ? sp.move_frac : sp.move_angle) * G[6 * p + k]);
+    const double nt = grid_floor(sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]));
+    const double nr = grid_floor(sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]));
+    for (int k = 0; k < 6; ++k) {
+      const double mv = k < 3 ? sp.move_frac : sp.move_angle;
+      const double n = k < 3 ? nt : nr;
+      const double step = n < mv ? g[k] : grid_floor(g[k] / n);
+      s1[6 * i + k] = x0[6 * p + k] - mv * step;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int d = direction[s] - 1;
+    s1[d] = x0[6 * base + d];
+    s1[6 * (N - 1) + d] = x0[6 * (base + N - 1) + d];
+    for (int k = 0; k < 6; ++k)
+      if (k != d) s1[6 * (N - 1) + k] = s1[k];
+    for (int i = 1; i < N - 1; ++i) {
+      int from = -1;
+      if (s1[6 * i + d] < 0.0) {
+        for (int ii = i + 1; ii < N; ++ii)
+          if (s1[6 * ii + d] > 0.0) { from = ii; break; }
+      } else if (s1[6 * i + d] > 1.0) {
+        for (int ii = i - 1; ii >= 0; --ii)
+          if (s1[6 * ii + d] < 1.0) { from = ii; break; }
+      }
+      if (from >= 0)
+        for (int k = 0; k < 6; ++k) s1[6 * i + k] = s1[6 * from + k];
+    }
+    // cumulative Cartesian arc length of (a, b, c)
+    cum[0] = 0.0;
+    double prev[3];
+    for (int k = 0; k < 3; ++k)
+      prev[k] = s1[0] * J.H[k] + s1[1] * J.H[3 + k] + s1[2] * J.H[6 + k];
+    for (int i = 1; i < N; ++i) {
+      double cur[3], l2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        cur[k] = s1[6 * i] * J.H[k] + s1[6 * i + 1] * J.H[3 + k] + s1[6 * i + 2] * J.H[6 + k];
+        l2 += (cur[k] - prev[k]) * (cur[k] - prev[k]);
+        prev[k] = cur[k];
+      }
+      cum[i] = cum[i - 1] + sqrt(l2);
+    }
+  }
+  __syncthreads();
+  const double total = cum[N - 1];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int p = base + i;
+    const double target = (double)i / (double)(N - 1);
+    int ii = 0;
+    if (total > 0.0) {
+      int lo = 0, hi = N - 2;   // last segment index whose start fraction <= target
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cum[mid] / total <= target) lo = mid; else hi = mid - 1;
+      }
+      ii = lo;
+    }
+    const double f0 = total > 0.0 ? cum[ii] / total : 0.0;
+    const double f1 = total > 0.0 ? cum[ii + 1] / total : 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double a0 = s1[6 * ii + k], a1 = s1[6 * (ii + 1) + k];
+      x2[6 * p + k] = (f1 > f0) ? a0 + (target - f0) * ((a1 - a0) / (f1 - f0)) : a0;
+    }
+    for (int k = 3; k < 6; ++k) x2[6 * p + k] = s1[6 * i + k];
+  }
+}
+
+// One periodic-aware smoothing of angle k of image i (global.h:1905-2085).
+__device__ double smooth_one(const double* ang, int i, int N, int k) {
+  const double c = ang[3 * i + k];
+  if (i == 0 || i == N - 1) return c;
+  const double l = ang[3 * (i - 1) + k], r = ang[3 * (i + 1) + k];
+  const double dl = l - c, dr = r - c;
+  double wraps;
+  if (dl > kPi) wraps = dr > kPi ? -2.0 : -1.0;
+  else if (dl < -kPi) wraps = dr < -kPi ? 2.0 : 0.0;
+  else wraps = dr > kPi ? -1.0 : (dr < -kPi ? 1.0 : 0.0);
+  return (1.0 - kSmooth) * c + 0.5 * kSmooth * (l + r + wraps * 2.0 * kPi);
+}
+
+// Steps 5-7: angle revert, smoothing, convergence test, commit.
+__global__ __launch_bounds__(256) void string_finish_kernel(StringParams sp,
+                                                            int* __restrict__ converged,
+                                                            int* __restrict__ iterations,
+                                                            double* __restrict__ x0,
+                                                            const double* __restrict__ x2,
+                                                            const double* __restrict__ V0,
+                                                            const double* __restrict__ V2) {
+  extern __shared__ double sh[];
+  const int s = blockIdx.x;
+  if (converged[s]) return;
+  const int N = sp.n_images;
+  double* ang = sh;                 // [N][3] checked s2 angles
+  double* red = sh + 3 * N;         // [2 * blockDim]
+  const int base = s * N;
+  double dc = 0.0, dor = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int p = base + i;
+    const bool revert = V2[p] > V0[p];
+    double sq = 0.0, so = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double a = revert ? x0[6 * p + 3 + k] : x2[6 * p + 3 + k];
+      ang[3 * i + k] = a;
+      const double e = x0[6 * p + k] - x2[6 * p + k];
+      const double q = x0[6 * p + 3 + k] - a;
+      sq += e * e;
+      so += q * q;
+    }
+    dc += sqrt(sq);
+    dor += sqrt(so);
+  }
+  red[threadIdx.x] = dc;
+  red[blockDim.x + threadIdx.x] = dor;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      red[blockDim.x + threadIdx.x] += red[blockDim.x + threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  const double tot_c = red[0], tot_o = red[blockDim.x];
+  const int it = iterations[s];
+  const bool check = it % kCheckEvery == 0 && it >= kCheckEvery;
+  const bool done = check && isfinite(tot_c) && isfinite(tot_o) && tot_c < sp.conv_c &&
+                    tot_o < sp.conv_o;
+  __syncthreads();
+  if (done) {
+    if (threadIdx.x == 0) converged[s] = 1;
+    return;
+  }
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int p = base + i;
+    for (int k = 0; k < 3; ++k) x0[6 * p + k] = x2[6 * p + k];
+    for (int k = 0; k < 3; ++k) {
+      double a = smooth_one(ang, i, N, k);
+      if (a > 2.0 * kPi) a -= 2.0 * kPi;
+      else if (a < 0.0) a += 2.0 * kPi;
+      x0[6 * p + 3 + k] = a;
+    }
+  }
+  if (threadIdx.x == 0) iterations[s] = it + 1;
+}
+
+hipError_t launch_image(int precision, bool grad, const Job* plans, const int* plan_of, int N,
+                        int P, const int* converged, const double* x, double* V, double* G,
+                        hipStream_t st) {
+  const dim3 g((P + 3) / 4), b(256);
+  if (precision == PREC_MIXED) {
+    if (grad) hipLaunchKernelGGL((image_kernel<true, true>), g, b, 0, st, plans, plan_of, N, P, converged, x, V, G);
+    else hipLaunchKernelGGL((image_kernel<true, false>), g, b, 0, st, plans, plan_of, N, P, converged, x, V, G);
+  } else {
+    if (grad) hipLaunchKernelGGL((image_kernel<false, true>), g, b, 0, st, plans, plan_of, N, P, converged, x, V, G);
+    else hipLaunchKernelGGL((image_kernel<false, false>), g, b, 0, st, plans, plan_of, N, P, converged, x, V, G);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+struct StringSys {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int S = 0, N = 0, P = 0, precision = 0;
+  StringParams sp{};
+  std::vector<Job> jobs;
+  DevBuf<Job> d_jobs;
+  DevBuf<int> d_plan_of, d_dir, d_conv, d_iter;
+  DevBuf<double> d_x0, d_x2, d_G, d_V0, d_V2;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~StringSys() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+#define SV_HIP_S(call)                                           \
+  do {                                                           \
+    hipError_t sv_e_ = (call);                                   \
+    if (sv_e_ != hipSuccess) return hip_fail(sv_e_, #call);      \
+  } while (0)
+
+int iteration(StringSys* Y) {
+  const hipStream_t st = Y->stream;
+  const size_t step_shm = sizeof(double) * 7 * Y->N;
+  const size_t fin_shm = sizeof(double) * (3 * Y->N + 2 * 256);
+  SV_HIP_S(launch_image(Y->precision, true, Y->d_jobs.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
+                        Y->d_x0.p, Y->d_V0.p, Y->d_G.p, st));
+  hipLaunchKernelGGL(string_step_kernel, dim3(Y->S), dim3(256), step_shm, st, Y->d_jobs.p,
+                     Y->d_plan_of.p, Y->d_dir.p, Y->sp, Y->d_conv.p, Y->d_x0.p, Y->d_G.p,
+                     Y->d_x2.p);
+  SV_HIP_S(hipGetLastError());
+  SV_HIP_S(launch_image(Y->precision, false, Y->d_jobs.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
+                        Y->d_x2.p, Y->d_V2.p, nullptr, st));
+  hipLaunchKernelGGL(string_finish_kernel, dim3(Y->S), dim3(256), fin_shm, st, Y->sp, Y->d_conv.p,
+                     Y->d_iter.p, Y->d_x0.p, Y->d_x2.p, Y->d_V0.p, Y->d_V2.p);
+  SV_HIP_S(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+}  // namespace sv
+
+using namespace sv;
+
+extern "C" {
+
+int sv_strings_create(int n_strings, void* const* plans, int n_images, const double* init,
+                      const int* direction, double move_frac, double move_angle_rad,
+                      int precision, void** strings) {
+  if (!plans || !init || !direction || !strings) return fail(1, "null argument");
+  if (n_strings < 1 || n_images < 3 || n_images > 1024) return fail(1, "1+ strings of 3..1024 images required");
+  if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
+  if (!(move_frac > 0.0 && move_angle_rad > 0.0 && std::isfinite(move_frac) && std::isfinite(move_angle_rad)))
+    return fail(1, "positive finite moves required");
+  for (int s = 0; s < n_strings; ++s)
+    if (direction[s] < 1 || direction[s] > 3 || !plans[s]) return fail(1, "direction must be 1, 2 or 3 and plans non-null");
+  for (long long i = 0; i < 6LL * n_strings * n_images; ++i)
+    if (!std::isfinite(init[i])) return fail(1, "nonfinite initial string");
+  auto Y = new StringSys();
+  Y->S = n_strings;
+  Y->N = n_images;
+  Y->P = n_strings * n_images;
+  Y->precision = precision;
+  Y->sp.n_strings = n_strings;
+  Y->sp.n_images = n_images;
+  Y->sp.move_frac = move_frac;
+  Y->sp.move_angle = move_angle_rad;
+  Y->sp.conv_c = 0.30 * move_frac * n_images * std::sqrt(3.0);
+  Y->sp.conv_o = 0.30 * move_angle_rad * n_images * std::sqrt(3.0);
+  std::map<const void*, int> index;
+  std::vector<int> plan_of(n_strings);
+  for (int s = 0; s < n_strings; ++s) {
+    auto it = index.find(plans[s]);
+    if (it == index.end()) {
+      Job J;
+      std::memset(&J, 0, sizeof(Job));
+      plan_fill_job(*static_cast<Plan*>(plans[s]), J);
+      it = index.emplace(plans[s], (int)Y->jobs.size()).first;
+      Y->jobs.push_back(J);
+    }
+    plan_of[s] = it->second;
+  }
+  Y->device = static_cast<Plan*>(plans[0])->device;
+  std::vector<int> zeros(n_strings, 0);
+  hipError_t e;
+  if ((e = hipSetDevice(Y->device)) != hipSuccess || (e = hipStreamCreate(&Y->stream)) != hipSuccess ||
+      (e = Y->d_jobs.upload(Y->jobs.data(), Y->jobs.size(), Y->stream)) != hipSuccess ||
+      (e = Y->d_plan_of.upload(plan_of.data(), n_strings, Y->stream)) != hipSuccess ||
+      (e = Y->d_dir.upload(direction, n_strings, Y->stream)) != hipSuccess ||
+      (e = Y->d_conv.upload(zeros.data(), n_strings, Y->stream)) != hipSuccess ||
+      (e = Y->d_iter.upload(zeros.data(), n_strings, Y->stream)) != hipSuccess ||
+      (e = Y->d_x0.upload(init, 6 * (size_t)Y->P, Y->stream)) != hipSuccess ||
+      (e = Y->d_x2.reserve(6 * (size_t)Y->P)) != hipSuccess ||
+      (e = Y->d_G.reserve(6 * (size_t)Y->P)) != hipSuccess ||
+      (e = Y->d_V0.reserve(Y->P)) != hipSuccess || (e = Y->d_V2.reserve(Y->P)) != hipSuccess ||
+      (e = hipStreamSynchronize(Y->stream)) != hipSuccess) {
+    delete Y;
+    return hip_fail(e, "string setup");
+  }
+  *strings = Y;
+  return 0;
+}
+
+int sv_strings_run(void* strings, int iterations, int use_graph) {
+  if (!strings || iterations < 0) return fail(1, "invalid argument");
+  StringSys* Y = static_cast<StringSys*>(strings);
+  SV_HIP_S(hipSetDevice(Y->device));
+  if (use_graph && !Y->exec) {
+    SV_HIP_S(hipStreamBeginCapture(Y->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = iteration(Y);
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(Y->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    SV_HIP_S(end);
+    Y->graph = g;
+    SV_HIP_S(hipGraphInstantiate(&Y->exec, Y->graph, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < iterations; ++i) {
+    if (use_graph) {
+      SV_HIP_S(hipGraphLaunch(Y->exec, Y->stream));
+    } else {
+      const int rc = iteration(Y);
+      if (rc) return rc;
+    }
+  }
+  SV_HIP_S(hipStreamSynchronize(Y->stream));
+  return 0;
+}
+
+int sv_strings_state(void* strings, double* path, double* energy, int* iterations, int* converged) {
+  if (!strings) return fail(1, "null strings");
+  StringSys* Y = static_cast<StringSys*>(strings);
+  SV_HIP_S(hipSetDevice(Y->device));
+  if (energy)
+    SV_HIP_S(launch_image(Y->precision, false, Y->d_jobs.p, Y->d_plan_of.p, Y->N, Y->P, nullptr,
+                          Y->d_x0.p, Y->d_V0.p, nullptr, Y->stream));
+  if (path) SV_HIP_S(hipMemcpyAsync(path, Y->d_x0.p, sizeof(double) * 6 * Y->P, hipMemcpyDeviceToHost, Y->stream));
+  if (energy) SV_HIP_S(hipMemcpyAsync(energy, Y->d_V0.p, sizeof(double) * Y->P, hipMemcpyDeviceToHost, Y->stream));
+  if (iterations) SV_HIP_S(hipMemcpyAsync(iterations, Y->d_iter.p, sizeof(int) * Y->S, hipMemcpyDeviceToHost, Y->stream));
+  if (converged) SV_HIP_S(hipMemcpyAsync(converged, Y->d_conv.p, sizeof(int) * Y->S, hipMemcpyDeviceToHost, Y->stream));
+  SV_HIP_S(hipStreamSynchronize(Y->stream));
+  if (energy)
+    for (int p = 0; p < Y->P; ++p)
+      if (!std::isfinite(energy[p])) return fail(4, "nonfinite string energy");
+  return 0;
+}
+
+int sv_strings_destroy(void* strings) {
+  delete static_cast<StringSys*>(strings);
+  return 0;
+}
+
+}  // extern "C"
