@@ -127,8 +127,211 @@ def cpu_baseline(case, centers, axes, sample):
             "query_seconds": query, "compute_seconds": compute}
 
 
+def gpu_error_metrics(ref, cand):
+    """error_metrics on device tensors (accessible: E < 2000 K and |E| < 1e4 K)."""
+    import torch
+    diff = (ref - cand).abs()
+    acc = (ref < 2000.0) & (ref.abs() < 1e4)
+    rel = diff / ref.abs().clamp(min=1.0)
+    n_acc = int(acc.sum())
+    return {"accessible_poses": n_acc,
+            "accessible_max_abs_error_K": float(diff[acc].max()) if n_acc else None,
+            "accessible_rms_error_K": float(diff[acc].pow(2).mean().sqrt()) if n_acc else None,
+            "all_max_relative_error_floor1K": float(rel.max())}
+
+
+def workload_ragged(args):
+    """100 frameworks x 2 guests, 60^3 x 64 fields each, one persistent launch."""
+    import torch
+    from string_vext_amd import autotune, synthetic
+    from string_vext_amd.field import Batch, Plan
+    t0 = time.perf_counter()
+    cases = synthetic.ragged_cases()
+    tuned = autotune.tune(cases)
+    log(f"autotune: {tuned['tile_centers']} blocks/CU {tuned['blocks_per_cu']} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    _, axes = __import__("string_vext_amd.orientation", fromlist=["x"]).regular_pose_grid(SIZE, ORIENTATIONS)
+    cost = [len(c["frame_frac"]) / np.linalg.det(c["cell"]) * (c["cutoff"] + c["bond_length"] / 2) ** 3
+            for c in cases]
+    order = np.argsort(cost)[::-1]                  # longest jobs first
+    plans = [Plan.from_case(cases[i]) for i in order]
+    jobs = [(p, p.grid(SIZE, axes, tile_centers=tuned["tile_centers"][autotune.density_class(cases[i])]))
+            for p, i in zip(plans, order)]
+    temps = [cases[i]["temperature"] for i in order]
+    log(f"{len(jobs)} fields set up ({time.perf_counter() - t0:.1f} s)")
+    batch = Batch(jobs, precision="mixed", temperatures=temps)
+    batch.set_persistent(tuned["blocks_per_cu"])
+    pairs = int(batch.census().sum())
+    log(f"census: {pairs} in-cutoff pair-evals over {len(jobs)} fields")
+    for _ in range(args.warmup):
+        batch.run(sync=True)
+    batch.sync()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(sync=False)
+    batch.sync()
+    step_s = (time.perf_counter() - t) / args.steps
+    batch.run(time_kernels=True, sync=True)
+    phases = batch.timings()
+    batch.close()
+    # per-field error of the mixed form against the FP64 form (chunks, metrics on the GPU)
+    per_field = []
+    if not args.no_check:
+        chunk = 10
+        for start in range(0, len(jobs), chunk):
+            part = jobs[start:start + chunk]
+            outs = {}
+            for prec in ("fp64", "mixed"):
+                tens = [torch.empty(ps.n_centers * ps.n_axes, dtype=torch.float64, device="cuda")
+                        for _, ps in part]
+                torch.cuda.synchronize()
+                b = Batch(part, precision=prec, outputs=(), external={"energy": [x.data_ptr() for x in tens]})
+                b.run(sync=True)
+                b.close()
+                outs[prec] = tens
+            for j in range(len(part)):
+                m = gpu_error_metrics(outs["fp64"][j], outs["mixed"][j])
+                m["field"] = int(order[start + j])
+                per_field.append(m)
+            del outs
+    acc = [m["accessible_max_abs_error_K"] for m in per_field if m["accessible_max_abs_error_K"] is not None]
+    line = {"metric": "pair-evals/s, ragged batch of 100 frameworks x 2 guests, 60^3x64 fields, mixed",
+            "value": pairs / step_s, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": step_s * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "ragged batch: 100 frameworks (200-4000 atoms log-uniform, "
+                                   "10-60 A, 60-120 deg) x 2 guests, 60^3 x 64 each, one persistent launch",
+                       "fields": len(jobs), "pair_evals": pairs, "autotune": tuned["tile_centers"],
+                       "blocks_per_cu": tuned["blocks_per_cu"]},
+            "phases_ms": phases,
+            "error_vs_fp64": {"fields_checked": len(per_field),
+                              "max_accessible_max_abs_error_K": max(acc) if acc else None,
+                              "fields_with_accessible_poses": len(acc),
+                              "max_relative_error_floor1K": max((m["all_max_relative_error_floor1K"] for m in per_field), default=None)},
+            "per_field": per_field}
+    print(json.dumps(line), flush=True)
+
+
+def workload_poses(args):
+    """8 full-size frameworks x 65,536 random poses: forward + backward through
+    torch.autograd, mixed and FP64, checked against FP64 and central differences."""
+    import torch
+    from string_vext_amd import synthetic
+    from string_vext_amd.autograd import pose_energy
+    from string_vext_amd.field import Plan
+    rng = np.random.default_rng(20261105)
+    results = {}
+    data = []
+    for k in range(8):
+        case = synthetic.full_field_case(seed=synthetic.FULL_SEED + k)
+        plan = Plan.from_case(case)
+        frac = rng.random((65536, 3))
+        u = rng.normal(size=(65536, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        c = torch.tensor(frac @ case["cell"], device="cuda")
+        data.append((plan, c, torch.tensor(u, device="cuda")))
+    for prec in ("mixed", "fp64"):
+        times, outs = [], []
+        for rep in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            step_out = []
+            for plan, c, u in data:
+                cc = c.clone().requires_grad_(True)
+                uu = u.clone().requires_grad_(True)
+                e = pose_energy(plan, cc, uu, prec)
+                e.sum().backward()
+                step_out.append((e.detach(), cc.grad, uu.grad))
+            torch.cuda.synchronize()
+            if rep >= args.warmup:
+                times.append(time.perf_counter() - t)
+            outs = step_out
+        results[prec] = (float(np.mean(times)), outs)
+    err_e, err_g = [], []
+    for (e64, gc64, gu64), (emx, gcmx, gumx) in zip(results["fp64"][1], results["mixed"][1]):
+        err_e.append(gpu_error_metrics(e64, emx))
+        scale = torch.maximum(gc64.abs().amax(dim=1, keepdim=True), torch.ones_like(gc64[:, :1]))
+        err_g.append(float(((gcmx - gc64).abs() / scale).max()))
+    # central differences (h = 1e-4 A) on a sample of accessible poses, FP64
+    plan, c, u = data[0]
+    e0 = results["fp64"][1][0][0]
+    idx = torch.nonzero(e0 < 2000).flatten()[:64]
+    h = 1e-4
+    fd_err = 0.0
+    for k in range(3):
+        step = torch.zeros(3, dtype=torch.float64, device="cuda")
+        step[k] = h
+        ep = pose_energy(plan, c[idx] + step, u[idx], "fp64")
+        em = pose_energy(plan, c[idx] - step, u[idx], "fp64")
+        fd = (ep - em) / (2 * h)
+        an = results["fp64"][1][0][1][idx, k]
+        fd_err = max(fd_err, float(((fd - an).abs() / an.abs().clamp(min=1.0)).max()))
+    n = 8 * 65536
+    line = {"metric": "poses/s, energy + dE/d(center, director) via torch.autograd, 8 x 65,536 poses",
+            "value": n / results["mixed"][0], "unit": "poses/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": results["mixed"][0] * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "8 full-size frameworks (1200 atoms, 37x37x14 A) "
+                                                         "x 65,536 uniform random poses, forward + backward"},
+            "fp64": {"value": n / results["fp64"][0], "ms_per_step": results["fp64"][0] * 1e3},
+            "speedup_mixed_vs_fp64": results["fp64"][0] / results["mixed"][0],
+            "energy_error_vs_fp64": {
+                "max_accessible_max_abs_error_K": max(m["accessible_max_abs_error_K"] or 0 for m in err_e),
+                "max_relative_error_floor1K": max(m["all_max_relative_error_floor1K"] for m in err_e)},
+            "gradient_error_vs_fp64_rel": max(err_g),
+            "central_difference_rel_error_h1e-4": fd_err}
+    print(json.dumps(line), flush=True)
+
+
+def workload_strings(args):
+    """64 strings x 96 images, 2000 iterations in a HIP graph, mixed vs FP64."""
+    from string_vext_amd import synthetic
+    from string_vext_amd.field import Plan
+    from string_vext_amd.strings import StringSystem, barrier, straight_strings
+    iters = args.iterations
+    plans, init, dirs = [], [], []
+    for k in range(8):
+        plan = Plan.from_case(synthetic.full_field_case(seed=synthetic.FULL_SEED + k))
+        for j in range(8):
+            d = 1 + (k * 8 + j) % 3
+            plans.append(plan)
+            dirs.append(d)
+            init.append(straight_strings(plan, d, 1, 96, seed=1000 * k + j)[0])
+    init = np.stack(init)
+    out = {}
+    for prec in ("mixed", "fp64"):
+        system = StringSystem(plans, init, np.array(dirs), precision=prec)
+        system.run(1, graph=True)                 # capture + first iteration
+        t = time.perf_counter()
+        system.run(iters - 1, graph=True)
+        dt = time.perf_counter() - t
+        out[prec] = (dt / (iters - 1), system.state())
+        log(f"strings {prec}: {dt / (iters - 1) * 1e3:.3f} ms/iteration")
+    p64, e64, it64, c64 = out["fp64"][1]
+    pmx, emx, itmx, cmx = out["mixed"][1]
+    b64, bmx = barrier(e64), barrier(emx)
+    acc = e64.max(axis=1) < 2000.0
+    line = {"metric": "string iterations/s, 64 strings x 96 images, hipGraph",
+            "value": 1.0 / out["mixed"][0], "unit": "iterations/s", "n_gpus": 1, "steps": iters,
+            "warmup": 1, "ms_per_step": out["mixed"][0] * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "64 strings x 96 images (8 full-size frameworks x 8 strings, "
+                                   "straight fractional paths along a/b/c), 2000 iterations"},
+            "fp64_ms_per_iteration": out["fp64"][0] * 1e3,
+            "strings_with_accessible_paths": int(acc.sum()),
+            "max_barrier_diff_mixed_vs_fp64_K_accessible": float(np.abs(b64 - bmx)[acc].max()) if acc.any() else None,
+            "max_profile_diff_K_accessible": float(np.abs(e64 - emx)[acc].max()) if acc.any() else None,
+            "converged_fp64": int(c64.sum()), "converged_mixed": int(cmx.sum()),
+            "barriers_fp64_K": b64.tolist(), "barriers_mixed_K": bmx.tolist()}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--workload", choices=("field", "ragged", "poses", "strings"), default="field",
+                    help="field (default): the 60^3 x 64 headline; the others measure the "
+                         "ragged batch, differentiable poses and batched strings")
+    ap.add_argument("--iterations", type=int, default=2000, help="strings workload iterations")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -147,6 +350,9 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
+    if args.workload != "field":
+        {"ragged": workload_ragged, "poses": workload_poses, "strings": workload_strings}[args.workload](args)
+        return
 
     from string_vext_amd import synthetic
     from string_vext_amd.field import Plan

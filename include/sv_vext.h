@@ -89,6 +89,13 @@ int sv_poses_create(void *plan, int n_centers, const double *centers_frac, int n
                     const double *axes, void **poses);
 int sv_poses_grid(void *plan, int size, int n_axes, const double *axes, void **poses);
 int sv_poses_destroy(void *poses);
+/* As above with the tile target: pose centers are binned into spatial tiles  */
+/* of about `tile_centers` centers that share one candidate list (the plain  */
+/* entry points use 64; string_vext_amd.autotune picks it per workload).     */
+int sv_poses_create_ex(void *plan, int n_centers, const double *centers_frac, int n_axes,
+                       const double *axes, int tile_centers, void **poses);
+int sv_poses_grid_ex(void *plan, int size, int n_axes, const double *axes, int tile_centers,
+                     void **poses);
 
 /* ------------------------------------------------------------------------ */
 /* Whole-field evaluation, host outputs (synchronous).                       */
@@ -128,6 +135,10 @@ int sv_batch_sync(void *batch);
 int sv_batch_timings(void *batch, double *ms7);
 int sv_batch_census(void *batch, unsigned long long *pairs_per_job);
 int sv_batch_destroy(void *batch);
+/* Mixed-precision batches only: launch the field as one persistent grid of   */
+/* blocks_per_cu x (number of CUs) blocks whose waves drain an XCD-partitioned */
+/* work queue (0 restores one block per 4 work items).                        */
+int sv_batch_set_persistent(void *batch, int blocks_per_cu);
 
 /* ------------------------------------------------------------------------ */
 /* Independent poses: pose i is (center_i, director_i); used for energies at */

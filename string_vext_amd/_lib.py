@@ -30,6 +30,8 @@ SIGNATURES = [
     ("sv_poses_create", _I, [_P, _I, _P, _I, _P, _P]),
     ("sv_poses_grid", _I, [_P, _I, _I, _P, _P]),
     ("sv_poses_destroy", _I, [_P]),
+    ("sv_poses_create_ex", _I, [_P, _I, _P, _I, _P, _I, _P]),
+    ("sv_poses_grid_ex", _I, [_P, _I, _I, _P, _I, _P]),
     ("sv_field_run", _I, [_P, _P, _I, _D, _P, _P, _P, _P, _P]),
     ("sv_field_census", _I, [_P, _P, _P]),
     ("sv_batch_create", _I, [_I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
@@ -38,6 +40,7 @@ SIGNATURES = [
     ("sv_batch_timings", _I, [_P, _P]),
     ("sv_batch_census", _I, [_P, _P]),
     ("sv_batch_destroy", _I, [_P]),
+    ("sv_batch_set_persistent", _I, [_P, _I]),
     ("sv_pose_list_create", _I, [_P, _I, _P, _P, _P]),
     ("sv_pose_list_eval", _I, [_P, _I, _P, _P, _P, _P]),
     ("sv_strings_create", _I, [_I, _P, _I, _P, _P, _D, _D, _I, _P]),

@@ -113,6 +113,11 @@ hipError_t launch_tile_fill(const Job* d_jobs, const Tile* d_tiles, int n_tiles_
 hipError_t launch_field(int precision, const Job* d_jobs, int n_jobs, const long long* d_job_work,
                         long long n_work, const long long* d_offsets, const float4* d_list32,
                         const double* d_list64, hipStream_t s, bool all_sym);
+hipError_t launch_field_persistent(const Job* d_jobs, int n_jobs, const long long* d_job_work,
+                                  long long n_work, unsigned long long* d_counters, int n_blocks,
+                                  const long long* d_offsets, const float4* d_list32,
+                                  const double* d_list64, hipStream_t s, bool all_sym);
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_census(const Job* d_jobs, int n_jobs, const long long* d_job_work,
                          long long n_work, const long long* d_offsets, const double* d_list64,
                          hipStream_t s);

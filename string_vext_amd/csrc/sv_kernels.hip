@@ -239,11 +239,9 @@ struct WaveCtx {
   bool k_ok;
 };
 
-__device__ __forceinline__ bool setup_wave(const Job* __restrict__ jobs, int n_jobs,
-                                           const long long* __restrict__ job_work,
-                                           long long n_work, int wv, WaveCtx& c) {
-  const long long w = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
-  if (w >= n_work) return false;
+__device__ __forceinline__ void setup_item(const Job* __restrict__ jobs, int n_jobs,
+                                           const long long* __restrict__ job_work, long long w,
+                                           WaveCtx& c) {
   const int j = find_job(job_work, n_jobs, w);
   c.J = jobs + j;
   const long long local = w - c.J->work_begin;
@@ -255,7 +253,40 @@ __device__ __forceinline__ bool setup_wave(const Job* __restrict__ jobs, int n_j
   c.k = c.group * 64 + lane;
   c.k_ok = c.k < c.J->n_axes;
   if (!c.k_ok) c.k = c.J->n_axes - 1;
+}
+
+__device__ __forceinline__ bool setup_wave(const Job* __restrict__ jobs, int n_jobs,
+                                           const long long* __restrict__ job_work,
+                                           long long n_work, int wv, WaveCtx& c) {
+  const long long w = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
+  if (w >= n_work) return false;
+  setup_item(jobs, n_jobs, job_work, w, c);
   return true;
+}
+
+// Work queue of the persistent launch: the item range is cut into 8
+// contiguous partitions, one per XCD (blocks are dispatched round-robin over
+// the XCDs, so block b starts on partition b % 8 and neighbouring items --
+// same tile, same candidate list -- stay in one L2).  A wave takes items from
+// its partition with one atomic per item and moves on to the next partition
+// when it is drained; after all 8 are drained it returns -1, so every wave
+// exits.
+__device__ __forceinline__ long long next_item(unsigned long long* __restrict__ counters,
+                                               long long n_work, int& part, int& visited) {
+  const long long q = n_work / 8, r = n_work % 8;
+  while (visited < 8) {
+    const long long size = q + (part < r ? 1 : 0);
+    const long long begin = part * q + (part < r ? part : r);
+    unsigned long long got = 0;
+    if ((threadIdx.x & 63) == 0) got = atomicAdd(&counters[part], 1ull);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)got);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(got >> 32));
+    const long long g = (long long)(((unsigned long long)hi << 32) | lo);
+    if (g < size) return begin + g;
+    part = (part + 1) & 7;
+    ++visited;
+  }
+  return -1;
 }
 
 // Pose energies, fused marginal/grid, and the site field at the center
@@ -329,20 +360,25 @@ __device__ __forceinline__ double pair64_d2(double d2, double rc2, double c12, d
   return d2 < rc2 ? e : 0.0;
 }
 
+// Atoms per FP32 partial sum in the mixed kernel.
+constexpr int kFlush = 16;
+
+struct MixedLds {
+  float4 in[4][64];
+  float4 far[4][64];
+  int near[4][64];
+};
+
+// One pose center x 64 orientations of the mixed-precision field.
 template <bool SYM>
-__global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict__ jobs, int n_jobs,
-                                                          const long long* __restrict__ job_work,
-                                                          long long n_work,
-                                                          const long long* __restrict__ offsets,
-                                                          const float4* __restrict__ list32,
-                                                          const double* __restrict__ list64) {
-  __shared__ float4 s_in[4][64];
-  __shared__ float4 s_far[4][64];
-  __shared__ int s_near[4][64];
+__device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L,
+                                           const long long* __restrict__ offsets,
+                                           const float4* __restrict__ list32,
+                                           const double* __restrict__ list64) {
+  float4 (*s_in)[64] = L.in;
+  float4 (*s_far)[64] = L.far;
+  int (*s_near)[64] = L.near;
   const int lane = threadIdx.x & 63;
-  const int wv = uniform(threadIdx.x >> 6);
-  WaveCtx c;
-  if (!setup_wave(jobs, n_jobs, job_work, n_work, wv, c)) return;
   const Job& J = *c.J;
   // The site field of a center is the same pair sum taken at the center
   // itself over the same candidates; the group-0 wave of the center
@@ -405,36 +441,49 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
       wave_sync();
       const int ni = __popcll(min_), nf = __popcll(mfar), nn = __popcll(mnear);
       kept_far += ni + nf;
-      float acc = 0.f, acc1 = 0.f;
+      // FP32 partials (one per site) cover at most kFlush atoms before they are
+      // added into the FP64 accumulator, so a partial never grows past a few
+      // hundred K and its rounding stays ~1e-5 K.
+      for (int q0 = 0; q0 < ni; q0 += kFlush) {
+        const int q1 = q0 + kFlush < ni ? q0 + kFlush : ni;
+        float acc = 0.f, acc1 = 0.f;
 #pragma unroll 4
-      for (int q = 0; q < ni; ++q) {
-        const float4 f = s_in[wv][q];
-        const float tdot = f.x * ux + f.y * uy + f.z * uz;
-        const float d0 = fmaf(tdot, m2o0, f.w);
-        const float d1 = fmaf(tdot, m2o1, SYM ? f.w : f.w + dR2);
-        const float i0 = __builtin_amdgcn_rcpf(d0);
-        const float i1 = __builtin_amdgcn_rcpf(d1);
-        const float i06 = i0 * i0 * i0;
-        const float i16 = i1 * i1 * i1;
-        acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
-        acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
+        for (int q = q0; q < q1; ++q) {
+          const float4 f = s_in[wv][q];
+          const float tdot = f.x * ux + f.y * uy + f.z * uz;
+          const float d0 = fmaf(tdot, m2o0, f.w);
+          const float d1 = fmaf(tdot, m2o1, SYM ? f.w : f.w + dR2);
+          const float i0 = __builtin_amdgcn_rcpf(d0);
+          const float i1 = __builtin_amdgcn_rcpf(d1);
+          const float i06 = i0 * i0 * i0;
+          const float i16 = i1 * i1 * i1;
+          acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
+          acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
+        }
+        E = fma(c6, (double)acc, E);
+        E = fma(c6, (double)acc1, E);
       }
+      for (int q0 = 0; q0 < nf; q0 += kFlush) {
+        const int q1 = q0 + kFlush < nf ? q0 + kFlush : nf;
+        float acc = 0.f, acc1 = 0.f;
 #pragma unroll 4
-      for (int q = 0; q < nf; ++q) {
-        const float4 f = s_far[wv][q];
-        const float tdot = f.x * ux + f.y * uy + f.z * uz;
-        float d0 = fmaf(tdot, m2o0, f.w);
-        float d1 = fmaf(tdot, m2o1, SYM ? f.w : f.w + dR2);
-        d0 = __builtin_amdgcn_fmed3f(d0, fl2f, rc2f);
-        d1 = __builtin_amdgcn_fmed3f(d1, fl2f, rc2f);
-        const float i0 = __builtin_amdgcn_rcpf(d0);
-        const float i1 = __builtin_amdgcn_rcpf(d1);
-        const float i06 = i0 * i0 * i0;
-        const float i16 = i1 * i1 * i1;
-        acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
-        acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
+        for (int q = q0; q < q1; ++q) {
+          const float4 f = s_far[wv][q];
+          const float tdot = f.x * ux + f.y * uy + f.z * uz;
+          float d0 = fmaf(tdot, m2o0, f.w);
+          float d1 = fmaf(tdot, m2o1, SYM ? f.w : f.w + dR2);
+          d0 = __builtin_amdgcn_fmed3f(d0, fl2f, rc2f);
+          d1 = __builtin_amdgcn_fmed3f(d1, fl2f, rc2f);
+          const float i0 = __builtin_amdgcn_rcpf(d0);
+          const float i1 = __builtin_amdgcn_rcpf(d1);
+          const float i06 = i0 * i0 * i0;
+          const float i16 = i1 * i1 * i1;
+          acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
+          acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
+        }
+        E = fma(c6, (double)acc, E);
+        E = fma(c6, (double)acc1, E);
       }
-      E = fma(c6, (double)(acc + acc1), E);
       for (int q = 0; q < nn; ++q) {
         // FP64 near-contact pair: one dot product serves both sites.
         const long long id = base + s_near[wv][q];
@@ -452,6 +501,42 @@ __global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict_
     S_shift = fma((double)kept_far, sh, S_shift);
   }
   write_pose(c, E, do_site, lane == 0 ? S + S_shift : S);
+}
+
+template <bool SYM>
+__global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict__ jobs, int n_jobs,
+                                                          const long long* __restrict__ job_work,
+                                                          long long n_work,
+                                                          const long long* __restrict__ offsets,
+                                                          const float4* __restrict__ list32,
+                                                          const double* __restrict__ list64) {
+  __shared__ MixedLds L;
+  const int wv = uniform(threadIdx.x >> 6);
+  WaveCtx c;
+  if (!setup_wave(jobs, n_jobs, job_work, n_work, wv, c)) return;
+  mixed_item<SYM>(c, wv, L, offsets, list32, list64);
+}
+
+// Persistent form: a fixed grid whose waves drain the XCD-partitioned queue.
+template <bool SYM>
+__global__ __launch_bounds__(256) void field_mixed_persistent(const Job* __restrict__ jobs,
+                                                              int n_jobs,
+                                                              const long long* __restrict__ job_work,
+                                                              long long n_work,
+                                                              unsigned long long* __restrict__ counters,
+                                                              const long long* __restrict__ offsets,
+                                                              const float4* __restrict__ list32,
+                                                              const double* __restrict__ list64) {
+  __shared__ MixedLds L;
+  const int wv = uniform(threadIdx.x >> 6);
+  int part = (int)(blockIdx.x % 8), visited = 0;
+  for (;;) {
+    const long long w = next_item(counters, n_work, part, visited);
+    if (w < 0) break;
+    WaveCtx c;
+    setup_item(jobs, n_jobs, job_work, w, c);
+    mixed_item<SYM>(c, wv, L, offsets, list32, list64);
+  }
 }
 
 __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__ jobs, int n_jobs,
@@ -855,6 +940,23 @@ hipError_t launch_field(int precision, const Job* d_jobs, int n_jobs, const long
     hipLaunchKernelGGL(field_fp64_kernel, grid, block, 0, s, d_jobs, n_jobs, d_job_work, n_work,
                        d_offsets, d_list64);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_field_persistent(const Job* d_jobs, int n_jobs, const long long* d_job_work,
+                                  long long n_work, unsigned long long* d_counters, int n_blocks,
+                                  const long long* d_offsets, const float4* d_list32,
+                                  const double* d_list64, hipStream_t s, bool all_sym) {
+  if (n_work <= 0) return hipSuccess;
+  hipError_t e = launch_zero_u64(d_counters, 8, s);
+  if (e != hipSuccess) return e;
+  const dim3 grid((unsigned)n_blocks), block(256);
+  if (all_sym)
+    hipLaunchKernelGGL(field_mixed_persistent<true>, grid, block, 0, s, d_jobs, n_jobs, d_job_work,
+                       n_work, d_counters, d_offsets, d_list32, d_list64);
+  else
+    hipLaunchKernelGGL(field_mixed_persistent<false>, grid, block, 0, s, d_jobs, n_jobs, d_job_work,
+                       n_work, d_counters, d_offsets, d_list32, d_list64);
   return hipGetLastError();
 }
 

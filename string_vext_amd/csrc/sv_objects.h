@@ -100,6 +100,8 @@ struct Batch {
   hipEvent_t ev[12] = {};
   bool timed_last = false;
   bool lists_built = false;
+  int persistent_blocks = 0;          // > 0: persistent field launch of this many blocks
+  DevBuf<unsigned long long> d_counters;
   ~Batch() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);

@@ -102,9 +102,9 @@ namespace {
 // Tile counts per lattice direction: tiles of roughly 64 centers, about as
 // thick along every direction.  A regular grid of `grid` points per axis gets
 // a divisor of `grid` so that tiles hold whole blocks of grid points.
-void choose_tiles(const Plan& P, long long n_centers, int grid, int m[3]) {
+void choose_tiles(const Plan& P, long long n_centers, int grid, int tile_centers, int m[3]) {
   const double rho = (double)n_centers / P.volume;
-  const double edge = std::cbrt(64.0 / std::max(rho, 1e-30));
+  const double edge = std::cbrt((double)tile_centers / std::max(rho, 1e-30));
   for (int k = 0; k < 3; ++k) {
     const double thick = 1.0 / P.hn[k];
     double want = thick / edge;
@@ -125,8 +125,9 @@ void choose_tiles(const Plan& P, long long n_centers, int grid, int m[3]) {
 }
 
 int build_poses(Plan* P, int n, const double* frac_in, int n_axes, const double* axes, int grid,
-                Poses** out) {
+                int tile_centers, Poses** out) {
   if (n < 0 || n_axes < 1) return fail(1, "n_centers >= 0 and n_axes >= 1 required");
+  if (tile_centers < 1 || tile_centers > 1 << 20) return fail(1, "tile_centers must be in 1..2^20");
   std::vector<double> f(3 * (size_t)n);
   for (size_t i = 0; i < 3 * (size_t)n; ++i) {
     if (!std::isfinite(frac_in[i])) return fail(1, "nonfinite center");
@@ -147,7 +148,7 @@ int build_poses(Plan* P, int n, const double* frac_in, int n_axes, const double*
     }
   }
   int m[3];
-  choose_tiles(*P, std::max(n, 1), grid, m);
+  choose_tiles(*P, std::max(n, 1), grid, tile_centers, m);
   const long long nbins = (long long)m[0] * m[1] * m[2];
   std::vector<int> bin(n);
   std::vector<long long> cnt(nbins + 1, 0);
@@ -367,8 +368,13 @@ int batch_run(Batch* B, bool timed, bool sync) {
   if (rc) return rc;
   SV_HIP(launch_zero(B->d_bad.p, 1, B->stream));
   if (timed) SV_HIP(hipEventRecord(B->ev[5], B->stream));
-  SV_HIP(launch_field(B->precision, B->d_jobs.p, B->n_jobs, B->d_job_work.p, B->n_work,
-                      B->d_offsets.p, B->d_list32.p, B->d_list64.p, B->stream, B->all_sym));
+  if (B->persistent_blocks > 0 && B->precision == PREC_MIXED)
+    SV_HIP(launch_field_persistent(B->d_jobs.p, B->n_jobs, B->d_job_work.p, B->n_work,
+                                   B->d_counters.p, B->persistent_blocks, B->d_offsets.p,
+                                   B->d_list32.p, B->d_list64.p, B->stream, B->all_sym));
+  else
+    SV_HIP(launch_field(B->precision, B->d_jobs.p, B->n_jobs, B->d_job_work.p, B->n_work,
+                        B->d_offsets.p, B->d_list32.p, B->d_list64.p, B->stream, B->all_sym));
   if (timed) SV_HIP(hipEventRecord(B->ev[6], B->stream));
   if (B->need_marginal)
     SV_HIP(launch_marginal(B->d_jobs.p, B->n_jobs, B->d_job_first.p, B->n_centers_total, B->stream));
@@ -383,6 +389,9 @@ int batch_run(Batch* B, bool timed, bool sync) {
   if (bad) return fail(4, "nonfinite Vext energy");
   return 0;
 }
+
+// Bound on the repulsive-wall slope (K/A) of pairs left to FP32 arithmetic.
+constexpr double kNearSlope = 300.0;
 
 int plan_create(const double* cell, int n_atoms, const double* frac, const double* sigma,
                 const double* epsilon, double gs, double ge, const double* offsets, double cutoff,
@@ -471,8 +480,18 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
     if (near_energy > 0.0) {
       // Repulsive-branch distance where 4 eps (x^2 - x) = near_energy, x = (sigma/r)^6.
       const double x = 0.5 * (1.0 + std::sqrt(1.0 + near_energy / me));
-      const double rstar = ms * std::pow(x, -1.0 / 6.0);
-      const double rn = rstar + omax;
+      const double r_energy = ms * std::pow(x, -1.0 / 6.0);
+      // ... and where the wall's slope |dphi/dr| = 24 eps/r (2 x^2 - x) drops to
+      // kNearSlope: an FP32 distance error of ~2e-7 A then costs <= 6e-5 K per
+      // term.  |dphi/dr| decreases monotonically on (0, 2^(1/6) sigma).
+      double lo = 0.5 * ms, hi = std::pow(2.0, 1.0 / 6.0) * ms;
+      for (int it = 0; it < 80; ++it) {
+        const double r = 0.5 * (lo + hi);
+        const double q = std::pow(ms / r, 6.0);
+        const double slope = 24.0 * me / r * (2.0 * q * q - q);
+        if (slope > kNearSlope) lo = r; else hi = r;
+      }
+      const double rn = std::max(r_energy, hi) + omax;
       near2 = (float)(rn * rn);
     }
     P->near2f.push_back(near2);
@@ -622,16 +641,27 @@ int sv_plan_info(void* plan, int* n_types, double* keep_radius) {
   return 0;
 }
 
-int sv_poses_create(void* plan, int n_centers, const double* centers_frac, int n_axes,
-                    const double* axes, void** poses) {
+int sv_poses_create_ex(void* plan, int n_centers, const double* centers_frac, int n_axes,
+                       const double* axes, int tile_centers, void** poses) {
   if (!plan || !poses || (n_centers > 0 && !centers_frac) || !axes) return fail(1, "null argument");
   Poses* S = nullptr;
-  int rc = build_poses(static_cast<Plan*>(plan), n_centers, centers_frac, n_axes, axes, 0, &S);
+  int rc = build_poses(static_cast<Plan*>(plan), n_centers, centers_frac, n_axes, axes, 0,
+                       tile_centers, &S);
   *poses = S;
   return rc;
 }
 
+int sv_poses_create(void* plan, int n_centers, const double* centers_frac, int n_axes,
+                    const double* axes, void** poses) {
+  return sv_poses_create_ex(plan, n_centers, centers_frac, n_axes, axes, 64, poses);
+}
+
 int sv_poses_grid(void* plan, int size, int n_axes, const double* axes, void** poses) {
+  return sv_poses_grid_ex(plan, size, n_axes, axes, 64, poses);
+}
+
+int sv_poses_grid_ex(void* plan, int size, int n_axes, const double* axes, int tile_centers,
+                     void** poses) {
   if (!plan || !poses || !axes) return fail(1, "null argument");
   if (size < 1 || (long long)size * size * size > 0x7fffffffLL) return fail(1, "invalid grid size");
   const long long n = (long long)size * size * size;
@@ -645,9 +675,25 @@ int sv_poses_grid(void* plan, int size, int n_axes, const double* axes, void** p
         f[3 * i + 2] = (ic + 0.5) / size;
       }
   Poses* S = nullptr;
-  int rc = build_poses(static_cast<Plan*>(plan), (int)n, f.data(), n_axes, axes, size, &S);
+  int rc = build_poses(static_cast<Plan*>(plan), (int)n, f.data(), n_axes, axes, size,
+                       tile_centers, &S);
   *poses = S;
   return rc;
+}
+
+int sv_batch_set_persistent(void* batch, int blocks_per_cu) {
+  if (!batch || blocks_per_cu < 0 || blocks_per_cu > 64) return fail(1, "invalid argument");
+  Batch* B = static_cast<Batch*>(batch);
+  if (blocks_per_cu == 0) {
+    B->persistent_blocks = 0;
+    return 0;
+  }
+  int cus = 0;
+  SV_HIP(hipSetDevice(B->device));
+  SV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->device));
+  SV_HIP(B->d_counters.reserve(8));
+  B->persistent_blocks = std::max(1, cus) * blocks_per_cu;
+  return 0;
 }
 
 int sv_poses_destroy(void* poses) {
@@ -805,7 +851,7 @@ int sv_pose_list_create(void* plan, int n_poses, const double* centers_cart, con
   }
   const double unit[3] = {1.0, 0.0, 0.0};
   Poses* S = nullptr;
-  int rc = build_poses(P, n_poses, frac.data(), 1, unit, 0, &S);
+  int rc = build_poses(P, n_poses, frac.data(), 1, unit, 0, 64, &S);
   if (rc) return rc;
   S->per_pose = true;
   S->pose_axes.resize(3 * (size_t)n_poses);

@@ -104,11 +104,11 @@ class Plan:
         _lib.check(self.lib.sv_plan_info(self.handle, ctypes.byref(n), None))
         return n.value
 
-    def poses(self, centers_frac, axes) -> "PoseSet":
-        return PoseSet(self, centers_frac=centers_frac, axes=axes)
+    def poses(self, centers_frac, axes, tile_centers=64) -> "PoseSet":
+        return PoseSet(self, centers_frac=centers_frac, axes=axes, tile_centers=tile_centers)
 
-    def grid(self, size, axes) -> "PoseSet":
-        return PoseSet(self, size=size, axes=axes)
+    def grid(self, size, axes, tile_centers=64) -> "PoseSet":
+        return PoseSet(self, size=size, axes=axes, tile_centers=tile_centers)
 
     def field(self, poses: "PoseSet", precision="fp64", temperature=None, energy=True,
               orientation=False, marginal=False, grid=False, site=False) -> dict:
@@ -150,7 +150,7 @@ class Plan:
 class PoseSet:
     """Centers x unit directors on the device (sv_poses_create / sv_poses_grid)."""
 
-    def __init__(self, plan: Plan, centers_frac=None, axes=None, size=None):
+    def __init__(self, plan: Plan, centers_frac=None, axes=None, size=None, tile_centers=64):
         self.lib = plan.lib
         self.plan = plan
         axes = _f64(axes)
@@ -158,16 +158,18 @@ class PoseSet:
             raise ValueError("axes must be (K, 3)")
         handle = ctypes.c_void_p()
         if size is not None:
-            _lib.check(self.lib.sv_poses_grid(plan.handle, int(size), len(axes), _ptr(axes),
-                                              ctypes.byref(handle)))
+            _lib.check(self.lib.sv_poses_grid_ex(plan.handle, int(size), len(axes), _ptr(axes),
+                                                 int(tile_centers), ctypes.byref(handle)))
             self.n_centers = int(size) ** 3
         else:
             centers = _f64(centers_frac)
             if centers.ndim != 2 or centers.shape[1] != 3:
                 raise ValueError("centers must be (n, 3) fractional coordinates")
-            _lib.check(self.lib.sv_poses_create(plan.handle, len(centers), _ptr(centers), len(axes),
-                                                _ptr(axes), ctypes.byref(handle)))
+            _lib.check(self.lib.sv_poses_create_ex(plan.handle, len(centers), _ptr(centers),
+                                                   len(axes), _ptr(axes), int(tile_centers),
+                                                   ctypes.byref(handle)))
             self.n_centers = len(centers)
+        self.tile_centers = int(tile_centers)
         self.handle = handle.value
         self.n_axes = len(axes)
 
@@ -188,16 +190,25 @@ class Batch:
     """
 
     def __init__(self, jobs, precision="mixed", temperatures=None,
-                 outputs=("orientation", "marginal", "grid", "site")):
+                 outputs=("orientation", "marginal", "grid", "site"), external=None):
+        """``external`` maps an output name to per-job device pointers owned by the
+        caller (e.g. torch ``data_ptr()`` values); those outputs are not allocated here."""
         if not jobs:
             raise ValueError("at least one (plan, poses) job required")
         self.lib = _lib.load()
         self.jobs = list(jobs)
         n = len(self.jobs)
+        external = external or {}
         self.buffers = {name: [] for name in ("energy", "orientation", "marginal", "grid", "site")}
         ptr_arrays = {}
         for name in self.buffers:
             ptrs = (ctypes.c_void_p * n)()
+            if name in external:
+                for j in range(n):
+                    ptrs[j] = int(external[name][j])
+                    self.buffers[name].append(None)
+                ptr_arrays[name] = ptrs
+                continue
             for j, (plan, poses) in enumerate(self.jobs):
                 if name in outputs:
                     count = poses.n_centers * (poses.n_axes if name in ("energy", "orientation") else 1)
@@ -209,6 +220,7 @@ class Batch:
                     self.buffers[name].append(None)
                     ptrs[j] = None
             ptr_arrays[name] = ptrs if name in outputs else None
+        self.precision = precision
         plans = (ctypes.c_void_p * n)(*[p.handle for p, _ in self.jobs])
         poses = (ctypes.c_void_p * n)(*[s.handle for _, s in self.jobs])
         temps = np.ascontiguousarray(
@@ -220,6 +232,10 @@ class Batch:
                                             ptr_arrays["marginal"], ptr_arrays["grid"],
                                             ptr_arrays["site"], ctypes.byref(handle)))
         self.handle = handle.value
+
+    def set_persistent(self, blocks_per_cu=8):
+        """Launch the mixed field as one persistent grid (0 disables)."""
+        _lib.check(self.lib.sv_batch_set_persistent(self.handle, int(blocks_per_cu)))
 
     def run(self, time_kernels=False, sync=True):
         _lib.check(self.lib.sv_batch_run(self.handle, int(bool(time_kernels)), int(bool(sync))))

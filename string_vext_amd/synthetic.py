@@ -10,7 +10,8 @@ Each generator returns a ``case`` dict with the keys of
                      (eps ~ U(8, 60) K, sigma ~ U(2.5, 3.8) A), 1.2 A rejection;
                      60^3 x 64 field
   ragged_cases       frameworks with N ~ logU(200, 4000), lengths ~ U(10, 60) A,
-                     angles ~ U(60, 120) (positive volume), x guests with
+                     angles ~ U(60, 120) (positive volume), uniform fractional
+                     coordinates, 4 LJ types as above, x guests with
                      eps ~ U(60, 250) K, sigma ~ U(3.3, 4.2) A, bond ~ U(1.2, 2.4) A
 
 Guest of the first two: two identical sites, eps 98 K, sigma 3.75 A, bond
@@ -101,9 +102,7 @@ def ragged_cases(n_frameworks=100, guests_per_framework=2, seed=RAGGED_SEED) -> 
             except ValueError:
                 continue
         n = int(round(np.exp(rng.uniform(np.log(200), np.log(4000)))))
-        volume = float(np.linalg.det(cell))
-        min_dist = min(1.2, 0.6 * (volume / n) ** (1.0 / 3.0))
-        frac = random_frame(rng, cell, n, min_dist=min_dist)
+        frac = rng.random((n, 3))
         type_eps = rng.uniform(8.0, 60.0, size=4)
         type_sig = rng.uniform(2.5, 3.8, size=4)
         types = rng.integers(0, 4, size=n)
