@@ -284,45 +284,68 @@ def workload_poses(args):
 
 
 def workload_strings(args):
-    """64 strings x 96 images, 2000 iterations in a HIP graph, mixed vs FP64."""
+    """64 strings x 96 images, 2000 iterations in a HIP graph, mixed vs FP64, on the
+    full-size frameworks and on a porous variant (400 atoms, same cell and types)."""
     from string_vext_amd import synthetic
     from string_vext_amd.field import Plan
     from string_vext_amd.strings import StringSystem, barrier, straight_strings
     iters = args.iterations
-    plans, init, dirs = [], [], []
-    for k in range(8):
-        plan = Plan.from_case(synthetic.full_field_case(seed=synthetic.FULL_SEED + k))
-        for j in range(8):
-            d = 1 + (k * 8 + j) % 3
-            plans.append(plan)
-            dirs.append(d)
-            init.append(straight_strings(plan, d, 1, 96, seed=1000 * k + j)[0])
-    init = np.stack(init)
-    out = {}
-    for prec in ("mixed", "fp64"):
-        system = StringSystem(plans, init, np.array(dirs), precision=prec)
-        system.run(1, graph=True)                 # capture + first iteration
-        t = time.perf_counter()
-        system.run(iters - 1, graph=True)
-        dt = time.perf_counter() - t
-        out[prec] = (dt / (iters - 1), system.state())
-        log(f"strings {prec}: {dt / (iters - 1) * 1e3:.3f} ms/iteration")
-    p64, e64, it64, c64 = out["fp64"][1]
-    pmx, emx, itmx, cmx = out["mixed"][1]
-    b64, bmx = barrier(e64), barrier(emx)
-    acc = e64.max(axis=1) < 2000.0
-    line = {"metric": "string iterations/s, 64 strings x 96 images, hipGraph",
-            "value": 1.0 / out["mixed"][0], "unit": "iterations/s", "n_gpus": 1, "steps": iters,
-            "warmup": 1, "ms_per_step": out["mixed"][0] * 1e3, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "64 strings x 96 images (8 full-size frameworks x 8 strings, "
-                                   "straight fractional paths along a/b/c), 2000 iterations"},
-            "fp64_ms_per_iteration": out["fp64"][0] * 1e3,
+    report = {}
+    for label, n_atoms in (("full", 1200), ("porous", 400)):
+        plans, init, dirs = [], [], []
+        for k in range(8):
+            plan = Plan.from_case(synthetic.full_field_case(seed=synthetic.FULL_SEED + k, n_atoms=n_atoms))
+            for j in range(8):
+                d = 1 + (k * 8 + j) % 3
+                plans.append(plan)
+                dirs.append(d)
+                init.append(straight_strings(plan, d, 1, 96, seed=1000 * k + j)[0])
+        init = np.stack(init)
+        out = {}
+        for prec in ("mixed", "fp64"):
+            system = StringSystem(plans, init, np.array(dirs), precision=prec)
+            system.run(1, graph=True)                 # capture + first iteration
+            t = time.perf_counter()
+            system.run(iters - 1, graph=True)
+            dt = time.perf_counter() - t
+            out[prec] = (dt / (iters - 1), system.state())
+            log(f"strings {label} {prec}: {dt / (iters - 1) * 1e3:.3f} ms/iteration")
+        p64, e64, it64, c64 = out["fp64"][1]
+        pmx, emx, itmx, cmx = out["mixed"][1]
+        b64, bmx = barrier(e64), barrier(emx)
+        # the FP64 run's final paths re-evaluated with the mixed-precision kernel
+        from string_vext_amd.field import PoseList
+        from string_vext_amd.strings import director
+        same = np.empty_like(e64)
+        for s_ in range(len(p64)):
+            pl = PoseList(plans[s_], p64[s_, :, :3] @ plans[s_].cell, director(p64[s_, :, 4], p64[s_, :, 5]))
+            same[s_] = pl.energy_host("mixed")
+        acc64 = e64.max(axis=1) < 2000.0
+        acc = (e64.max(axis=1) < 2000.0) & (emx.max(axis=1) < 2000.0)
+        rel = np.abs(b64 - bmx) / np.maximum(np.abs(b64), 1.0)
+        report[label] = {
+            "ms_per_iteration_mixed": out["mixed"][0] * 1e3,
+            "ms_per_iteration_fp64": out["fp64"][0] * 1e3,
             "strings_with_accessible_paths": int(acc.sum()),
-            "max_barrier_diff_mixed_vs_fp64_K_accessible": float(np.abs(b64 - bmx)[acc].max()) if acc.any() else None,
+            "max_barrier_diff_K_accessible": float(np.abs(b64 - bmx)[acc].max()) if acc.any() else None,
             "max_profile_diff_K_accessible": float(np.abs(e64 - emx)[acc].max()) if acc.any() else None,
-            "converged_fp64": int(c64.sum()), "converged_mixed": int(cmx.sum()),
-            "barriers_fp64_K": b64.tolist(), "barriers_mixed_K": bmx.tolist()}
+            "median_relative_barrier_diff_all": float(np.median(rel)),
+            "same_path_strings_accessible": int(acc64.sum()),
+            "same_path_max_profile_diff_K": float(np.abs(same - e64)[acc64].max()) if acc64.any() else None,
+            "same_path_max_barrier_diff_K": float(np.abs(barrier(same) - b64)[acc64].max()) if acc64.any() else None,
+            "iterations_fp64": it64.tolist(), "converged_fp64": int(c64.sum()),
+            "converged_mixed": int(cmx.sum()),
+            "barriers_fp64_K": [round(float(v), 6) for v in b64],
+            "barriers_mixed_K": [round(float(v), 6) for v in bmx]}
+    line = {"metric": "string iterations/s, 64 strings x 96 images, hipGraph",
+            "value": 1e3 / report["full"]["ms_per_iteration_mixed"], "unit": "iterations/s",
+            "n_gpus": 1, "steps": iters, "warmup": 1,
+            "ms_per_step": report["full"]["ms_per_iteration_mixed"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "64 strings x 96 images (8 frameworks x 8 strings, straight "
+                                   "fractional paths along a/b/c), 2000 iterations, one iteration "
+                                   "per HIP graph replay"},
+            "frameworks": report}
     print(json.dumps(line), flush=True)
 
 
@@ -360,8 +383,12 @@ def main():
 
     case = synthetic.full_field_case()
     centers, axes = regular_pose_grid(SIZE, ORIENTATIONS)
+    from string_vext_amd import autotune
+    tuned = autotune.tune([case], size=SIZE, orientations=ORIENTATIONS, reps=2, persistent=False)
+    tile_centers = tuned["tile_centers"][autotune.density_class(case)]
+    log(f"autotune: tile_centers={tile_centers} trials={tuned['trials']}")
     plan = Plan.from_case(case)
-    poses = plan.grid(SIZE, axes)
+    poses = plan.grid(SIZE, axes, tile_centers=tile_centers)
     pairs = plan.census(poses)
     log(f"census: {pairs} in-cutoff pair-evals per field ({pairs / (SIZE**3 * ORIENTATIONS):.1f} per pose)")
 
@@ -410,7 +437,8 @@ def main():
                                "1200-atom triclinic framework 37x37x14 A gamma 120, cutoff 12.8 A, "
                                "mixed FP32 pair / FP64 accumulate",
                    "grid": SIZE, "orientations": ORIENTATIONS, "framework_atoms": 1200,
-                   "pair_evals_per_field": pairs, "accumulate": "f64"},
+                   "pair_evals_per_field": pairs, "accumulate": "f64",
+                   "tile_centers": tile_centers},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "field_mixed_kernel", "kernel_ms": field_ms,

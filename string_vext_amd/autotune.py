@@ -51,7 +51,7 @@ def _time_batch(jobs, precision, blocks_per_cu, reps):
 
 
 def tune(cases, size=24, orientations=64, precision="mixed", reps=3, use_cache=True,
-         device_key="gpu0"):
+         device_key="gpu0", persistent=True):
     """Pick (tile_centers, blocks_per_cu) for a set of cases.
 
     Returns ``{"tile_centers": {density_class: n}, "blocks_per_cu": b, "trials": [...]}``.
@@ -69,7 +69,7 @@ def tune(cases, size=24, orientations=64, precision="mixed", reps=3, use_cache=T
         classes.setdefault(density_class(case), []).append(case)
     result = {"tile_centers": {}, "trials": []}
     for cls, members in sorted(classes.items()):
-        key = f"{device_key}/{precision}/{cls}"
+        key = f"{device_key}/{precision}/{cls}/n{size}k{orientations}"
         if key in cache:
             result["tile_centers"][cls] = cache[key]
             continue
@@ -83,10 +83,10 @@ def tune(cases, size=24, orientations=64, precision="mixed", reps=3, use_cache=T
         best = min(timing, key=timing.get)
         result["tile_centers"][cls] = best
         cache[key] = best
-    bkey = f"{device_key}/{precision}/blocks_per_cu"
+    bkey = f"{device_key}/{precision}/blocks_per_cu/n{size}k{orientations}"
     if bkey in cache:
         result["blocks_per_cu"] = cache[bkey]
-    elif precision == "mixed":
+    elif precision == "mixed" and persistent:
         plans = [Plan.from_case(c) for c in cases[: min(len(cases), 6)]]
         jobs = [(p, p.grid(size, axes, tile_centers=result["tile_centers"][density_class(c)]))
                 for p, c in zip(plans, cases)]

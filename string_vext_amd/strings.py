@@ -81,29 +81,35 @@ def barrier(energy):
 
 
 def straight_strings(plan: Plan, direction: int, n_strings: int, n_images: int, seed: int,
-                     scan: int = 24, precision: str = "fp64") -> np.ndarray:
+                     scan: int = 24, orientations: int = 8, probes: int = 16,
+                     precision: str = "fp64") -> np.ndarray:
     """Straight fractional paths along lattice direction 1/2/3 (a/b/c).
 
-    The starting point of each string is the lowest-energy pose of a seeded
-    coarse scan of the plane perpendicular to the hop direction (scan^2
-    positions x 8 orientations), as the reference solver seeds its string from
-    a plane scan minimum (RC_main.cu:622-802); the string then runs straight
-    along the hop direction from 0 to 1 with fixed orientation.
+    Each string is the straight line along the hop direction (from 0 to 1) with
+    the lowest maximum energy among a seeded scan of scan^2 positions of the
+    perpendicular plane x `orientations` fixed orientations, each line probed
+    at `probes` points -- the reference solver likewise seeds its string from a
+    plane scan (RC_main.cu:622-802) and a straight line (global.h:307-358).
     """
     rng = np.random.default_rng(seed)
     d = direction - 1
     others = [k for k in range(3) if k != d]
     out = np.zeros((n_strings, n_images, 6))
+    t = np.linspace(0.0, 1.0, probes)
     for s in range(n_strings):
         grid = rng.random((scan * scan, 2))
-        angles = np.c_[np.zeros(8), rng.uniform(0, np.pi, 8), rng.uniform(0, 2 * np.pi, 8)]
-        frac = np.zeros((len(grid) * 8, 3))
-        frac[:, others[0]] = np.repeat(grid[:, 0], 8)
-        frac[:, others[1]] = np.repeat(grid[:, 1], 8)
+        angles = np.c_[np.zeros(orientations), rng.uniform(0, np.pi, orientations),
+                       rng.uniform(0, 2 * np.pi, orientations)]
+        n_lines = len(grid) * orientations
+        frac = np.zeros((n_lines, probes, 3))
+        frac[:, :, others[0]] = np.repeat(grid[:, 0], orientations)[:, None]
+        frac[:, :, others[1]] = np.repeat(grid[:, 1], orientations)[:, None]
+        frac[:, :, d] = t[None, :]
         ang = np.tile(angles, (len(grid), 1))
-        e = PoseList(plan, frac @ plan.cell, director(ang[:, 1], ang[:, 2])).energy_host(precision)
-        best = int(np.argmin(e))
-        out[s, :, :3] = frac[best]
+        dirs = np.repeat(director(ang[:, 1], ang[:, 2]), probes, axis=0)
+        e = PoseList(plan, frac.reshape(-1, 3) @ plan.cell, dirs).energy_host(precision)
+        best = int(np.argmin(e.reshape(n_lines, probes).max(axis=1)))
+        out[s, :, :3] = frac[best, 0]
         out[s, :, d] = np.linspace(0.0, 1.0, n_images)
         out[s, :, 3:] = ang[best]
     return out

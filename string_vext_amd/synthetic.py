@@ -78,14 +78,16 @@ def small_field_case(seed=SMALL_SEED) -> dict:
     return make_case(lengths, angles, frac, sig, eps)
 
 
-def full_field_case(seed=FULL_SEED) -> dict:
+def full_field_case(seed=FULL_SEED, n_atoms=1200) -> dict:
+    """The full-size framework (``n_atoms`` < 1200 gives a porous variant of
+    the same cell and LJ types)."""
     rng = np.random.default_rng(seed)
     lengths, angles = (37.0, 37.0, 14.0), (90.0, 90.0, 120.0)
     cell = cell_matrix(lengths, angles)
-    frac = random_frame(rng, cell, 1200)
+    frac = random_frame(rng, cell, n_atoms)
     type_eps = rng.uniform(8.0, 60.0, size=4)
     type_sig = rng.uniform(2.5, 3.8, size=4)
-    types = rng.integers(0, 4, size=1200)
+    types = rng.integers(0, 4, size=n_atoms)
     return make_case(lengths, angles, frac, type_sig[types], type_eps[types])
 
 
