@@ -242,13 +242,15 @@ struct WaveCtx {
 __device__ __forceinline__ void setup_item(const Job* __restrict__ jobs, int n_jobs,
                                            const long long* __restrict__ job_work, long long w,
                                            WaveCtx& c) {
-  const int j = find_job(job_work, n_jobs, w);
+  // Every value here is wave-uniform; readfirstlane lets the compiler keep the
+  // job's fields in scalar registers also in the persistent kernel's loop.
+  const int j = uniform(find_job(job_work, n_jobs, w));
   c.J = jobs + j;
   const long long local = w - c.J->work_begin;
   const int ng = c.J->n_groups;
-  c.slot = (int)(local / ng);
-  c.group = (int)(local - (long long)c.slot * ng);
-  c.tile = c.J->tile_of[c.slot];
+  c.slot = uniform((int)(local / ng));
+  c.group = uniform((int)(local - (long long)c.slot * ng));
+  c.tile = uniform(c.J->tile_of[c.slot]);
   const int lane = threadIdx.x & 63;
   c.k = c.group * 64 + lane;
   c.k_ok = c.k < c.J->n_axes;
@@ -504,12 +506,10 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
 }
 
 template <bool SYM>
-__global__ __launch_bounds__(256) void field_mixed_kernel(const Job* __restrict__ jobs, int n_jobs,
-                                                          const long long* __restrict__ job_work,
-                                                          long long n_work,
-                                                          const long long* __restrict__ offsets,
-                                                          const float4* __restrict__ list32,
-                                                          const double* __restrict__ list64) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void field_mixed_kernel(
+    const Job* __restrict__ jobs, int n_jobs, const long long* __restrict__ job_work,
+    long long n_work, const long long* __restrict__ offsets, const float4* __restrict__ list32,
+    const double* __restrict__ list64) {
   __shared__ MixedLds L;
   const int wv = uniform(threadIdx.x >> 6);
   WaveCtx c;
@@ -651,6 +651,7 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
     const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
+    int n_in = 0;   // in-cutoff FP32 pairs of this type
     for (long long base = beg; base < end; base += 64) {
       const long long idx = base + lane;
       if (idx < end) {
@@ -664,54 +665,60 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
       wave_sync();
       const int n = (int)(end - base < 64 ? end - base : 64);
       if (MIXED) {
-        float aE = 0.f, aG0 = 0.f, aG1 = 0.f, aRc0 = 0.f, aRc1 = 0.f, aRc2 = 0.f, aRu0 = 0.f,
-              aRu1 = 0.f, aRu2 = 0.f;
-        for (int q = 0; q < n; ++q) {
-          const float4 e = s_e[q];
-          const float rx = e.x - cxf, ry = e.y - cyf, rz = e.z - czf;
-          const float rr = rx * rx + ry * ry + rz * rz;
-          const float tt = rx * uxf + ry * uyf + rz * uzf;
-          const bool near = rr < near2;
-          const float d0r = fmaf(tt, m2o0, rr + o0sq);
-          const float d1r = fmaf(tt, m2o1, rr + o1sq);
-          // a near lane contributes exactly -shift here (clamped to rc^2) and the
-          // FP64 reference term below
-          const float d0 = near ? rc2f : __builtin_amdgcn_fmed3f(d0r, fl2f, rc2f);
-          const float d1 = near ? rc2f : __builtin_amdgcn_fmed3f(d1r, fl2f, rc2f);
-          const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1);
-          const float i03 = i0 * i0 * i0, i13 = i1 * i1 * i1;
-          aE = fmaf(i03, fmaf(i03, s6, -1.0f), aE);
-          aE = fmaf(i13, fmaf(i13, s6, -1.0f), aE);
+        // FP32 partials over at most 16 atoms, then FP64.  Only in-cutoff FP32
+        // pairs contribute (and are counted for the shift), so candidates far
+        // outside the cutoff add nothing, not a rounded -shift.
+        for (int q0 = 0; q0 < n; q0 += 16) {
+          const int q1 = q0 + 16 < n ? q0 + 16 : n;
+          float aE = 0.f, aG0 = 0.f, aG1 = 0.f, aRc0 = 0.f, aRc1 = 0.f, aRc2 = 0.f, aRu0 = 0.f,
+                aRu1 = 0.f, aRu2 = 0.f;
+          for (int q = q0; q < q1; ++q) {
+            const float4 e = s_e[q];
+            const float rx = e.x - cxf, ry = e.y - cyf, rz = e.z - czf;
+            const float rr = rx * rx + ry * ry + rz * rz;
+            const float tt = rx * uxf + ry * uyf + rz * uzf;
+            const bool near = rr < near2;
+            const float d0r = fmaf(tt, m2o0, rr + o0sq);
+            const float d1r = fmaf(tt, m2o1, rr + o1sq);
+            const bool in0 = !near && d0r < rc2f;
+            const bool in1 = !near && d1r < rc2f;
+            const float d0 = __builtin_amdgcn_fmed3f(d0r, fl2f, rc2f);
+            const float d1 = __builtin_amdgcn_fmed3f(d1r, fl2f, rc2f);
+            const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1);
+            const float i03 = i0 * i0 * i0, i13 = i1 * i1 * i1;
+            aE += in0 ? i03 * fmaf(i03, s6, -1.0f) : 0.0f;
+            aE += in1 ? i13 * fmaf(i13, s6, -1.0f) : 0.0f;
+            n_in += (int)in0 + (int)in1;
+            if (GRAD) {
+              const bool f0 = in0 && d0r > fl2f, f1 = in1 && d1r > fl2f;
+              const float g0 = f0 ? i03 * i0 * fmaf(-2.0f * s6, i03, 1.0f) : 0.0f;
+              const float g1 = f1 ? i13 * i1 * fmaf(-2.0f * s6, i13, 1.0f) : 0.0f;
+              aG0 += g0;
+              aG1 += g1;
+              const float w = g0 + g1, v = fmaf(o0f, g0, o1f * g1);
+              aRc0 = fmaf(w, rx, aRc0); aRc1 = fmaf(w, ry, aRc1); aRc2 = fmaf(w, rz, aRc2);
+              aRu0 = fmaf(v, rx, aRu0); aRu1 = fmaf(v, ry, aRu1); aRu2 = fmaf(v, rz, aRu2);
+            }
+            if (near) {
+              const long long id = base + q;
+              const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
+                           qz = list64[3 * id + 2] - cz;
+              pose_pair64(o0, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
+              pose_pair64(o1, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
+            }
+          }
+          a.E = fma(c6, (double)aE, a.E);
           if (GRAD) {
-            const bool in0 = !near && d0r > fl2f && d0r < rc2f;
-            const bool in1 = !near && d1r > fl2f && d1r < rc2f;
-            const float g0 = in0 ? i03 * i0 * fmaf(-2.0f * s6, i03, 1.0f) : 0.0f;
-            const float g1 = in1 ? i13 * i1 * fmaf(-2.0f * s6, i13, 1.0f) : 0.0f;
-            aG0 += g0;
-            aG1 += g1;
-            const float w = g0 + g1, v = fmaf(o0f, g0, o1f * g1);
-            aRc0 = fmaf(w, rx, aRc0); aRc1 = fmaf(w, ry, aRc1); aRc2 = fmaf(w, rz, aRc2);
-            aRu0 = fmaf(v, rx, aRu0); aRu1 = fmaf(v, ry, aRu1); aRu2 = fmaf(v, rz, aRu2);
+            const double k = 3.0 * c6;
+            a.G0 = fma(k, (double)aG0, a.G0);
+            a.G1 = fma(k, (double)aG1, a.G1);
+            a.Rc[0] = fma(k, (double)aRc0, a.Rc[0]);
+            a.Rc[1] = fma(k, (double)aRc1, a.Rc[1]);
+            a.Rc[2] = fma(k, (double)aRc2, a.Rc[2]);
+            a.Ru[0] = fma(k, (double)aRu0, a.Ru[0]);
+            a.Ru[1] = fma(k, (double)aRu1, a.Ru[1]);
+            a.Ru[2] = fma(k, (double)aRu2, a.Ru[2]);
           }
-          if (near) {
-            const long long id = base + q;
-            const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
-                         qz = list64[3 * id + 2] - cz;
-            pose_pair64(o0, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
-            pose_pair64(o1, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
-          }
-        }
-        a.E = fma(c6, (double)aE, a.E);
-        if (GRAD) {
-          const double k = 3.0 * c6;
-          a.G0 = fma(k, (double)aG0, a.G0);
-          a.G1 = fma(k, (double)aG1, a.G1);
-          a.Rc[0] = fma(k, (double)aRc0, a.Rc[0]);
-          a.Rc[1] = fma(k, (double)aRc1, a.Rc[1]);
-          a.Rc[2] = fma(k, (double)aRc2, a.Rc[2]);
-          a.Ru[0] = fma(k, (double)aRu0, a.Ru[0]);
-          a.Ru[1] = fma(k, (double)aRu1, a.Ru[1]);
-          a.Ru[2] = fma(k, (double)aRu2, a.Ru[2]);
         }
       } else {
         for (int q = 0; q < n; ++q) {
@@ -722,8 +729,7 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
       }
       wave_sync();
     }
-    // every list entry went through the clamped FP32 term of both sites
-    if (MIXED) a.E = fma((double)(2 * (end - beg)), sh, a.E);
+    if (MIXED) a.E = fma((double)n_in, sh, a.E);
   }
   if (!ok) return;
   const int cid = J.center_id[slot];
