@@ -160,6 +160,8 @@ def workload_ragged(args):
     temps = [cases[i]["temperature"] for i in order]
     log(f"{len(jobs)} fields set up ({time.perf_counter() - t0:.1f} s)")
     batch = Batch(jobs, precision="mixed", temperatures=temps)
+    if args.blocks_per_cu is not None:
+        tuned["blocks_per_cu"] = args.blocks_per_cu
     batch.set_persistent(tuned["blocks_per_cu"])
     pairs = int(batch.census().sum())
     log(f"census: {pairs} in-cutoff pair-evals over {len(jobs)} fields")
@@ -362,6 +364,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="grid centers in the CPU baseline sample (0 skips it)")
     ap.add_argument("--no-check", action="store_true", help="skip the mixed-vs-FP64 error pass")
+    ap.add_argument("--blocks-per-cu", type=int, default=None,
+                    help="ragged workload: persistent grid size (0 = plain grid; default autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain elementwise copy of MIB MiB (FP32 in, FP32 out) so that "
                          "PMC byte counters can be scaled against a kernel of known traffic")

@@ -159,6 +159,26 @@ int sv_pose_list_eval(void *poses, int precision, const double *grad_out, double
                       double *grad_center, double *grad_axis);
 
 /* ------------------------------------------------------------------------ */
+/* Device-resident independent poses (no counterpart in the reference; for   */
+/* callers whose poses already live on the GPU, e.g. torch.autograd).        */
+/* sv_pose_grid_create fixes a grid of spatial tiles over the cell, sized    */
+/* for about n_expected poses at tile_centers per tile (0 = 64), and builds  */
+/* every tile's candidate list once.  sv_pose_grid_eval takes DEVICE arrays  */
+/* centers_cart [n][3] and axes [n][3] (FP64), bins the poses into the tiles */
+/* on the GPU and evaluates them exactly like sv_pose_list_eval (same        */
+/* outputs, all DEVICE pointers, each may be NULL).  Any n >= 1 is accepted  */
+/* (buffers grow).  Synchronous; returns 1 for a nonfinite center or a       */
+/* nonfinite/zero director, 4 for nonfinite results.                         */
+/* sv_pose_grid_info reports the tile count and the current pose capacity.   */
+/* ------------------------------------------------------------------------ */
+int sv_pose_grid_create(void *plan, int n_expected, int tile_centers, void **grid);
+int sv_pose_grid_eval(void *grid, int n, const double *centers_cart, const double *axes,
+                      int precision, const double *grad_out, double *energy,
+                      double *grad_center, double *grad_axis);
+int sv_pose_grid_info(void *grid, int *n_tiles, int *capacity);
+int sv_pose_grid_destroy(void *grid);
+
+/* ------------------------------------------------------------------------ */
 /* Batched String method: n_strings independent strings of n_images images. */
 /* Image state x = (a, b, c, alpha, beta, gamma): fractional center and      */
 /* Euler angles (rad) of the body-x guest, director (cos g cos b,            */

@@ -43,6 +43,10 @@ SIGNATURES = [
     ("sv_batch_set_persistent", _I, [_P, _I]),
     ("sv_pose_list_create", _I, [_P, _I, _P, _P, _P]),
     ("sv_pose_list_eval", _I, [_P, _I, _P, _P, _P, _P]),
+    ("sv_pose_grid_create", _I, [_P, _I, _I, _P]),
+    ("sv_pose_grid_eval", _I, [_P, _I, _P, _P, _I, _P, _P, _P, _P]),
+    ("sv_pose_grid_info", _I, [_P, _P, _P]),
+    ("sv_pose_grid_destroy", _I, [_P]),
     ("sv_strings_create", _I, [_I, _P, _I, _P, _P, _D, _D, _I, _P]),
     ("sv_strings_run", _I, [_P, _I, _I]),
     ("sv_strings_state", _I, [_P, _P, _P, _P, _P]),

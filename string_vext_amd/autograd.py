@@ -4,18 +4,29 @@
     E.sum().backward()                                        # centers.grad, axes.grad
 
 ``centers`` are Cartesian (Angstrom), ``axes`` the directors, both (n, 3)
-float64 CUDA tensors; directors are used as given (normalize them in torch if
-they are free parameters, and autograd carries the normalization).  Forward
-runs the energy kernel; backward runs a separate kernel that recomputes every
-pair term and returns dE/dc and dE/du scaled by the upstream gradient.  There
-is no counterpart in the reference, whose String forces are forward
-differences of the energy (native/string/source_code/global.h:1281-1324).
+CUDA tensors; directors are used as given (normalize them in torch if they are
+free parameters, and autograd carries the normalization).  The poses stay on
+the device: each call bins them into the plan's cached tile grid on the GPU
+(sv_pose_grid_eval).  Forward runs the energy kernel; backward runs a separate
+kernel that recomputes every pair term and returns dE/dc and dE/du scaled by
+the upstream gradient.  There is no counterpart in the reference, whose String
+forces are forward differences of the energy
+(native/string/source_code/global.h:1281-1324).
 """
 from __future__ import annotations
 
 import torch
 
-from .field import Plan, PoseList
+from .field import Plan, PoseGrid
+
+
+def _grid_for(plan: Plan, n: int) -> PoseGrid:
+    """The plan's pose grid for batches of about n poses (one per power of 2)."""
+    grids = plan.__dict__.setdefault("_pose_grids", {})
+    key = max(1, int(n)).bit_length()
+    if key not in grids:
+        grids[key] = PoseGrid(plan, 1 << (key - 1))
+    return grids[key]
 
 
 class PoseEnergy(torch.autograd.Function):
@@ -23,27 +34,31 @@ class PoseEnergy(torch.autograd.Function):
     def forward(ctx, centers, axes, plan: Plan, precision: str):
         if centers.shape != axes.shape or centers.ndim != 2 or centers.shape[1] != 3:
             raise ValueError("centers and axes must both be (n, 3)")
+        if len(centers) == 0:
+            raise ValueError("at least one pose required")
         device = centers.device if centers.is_cuda else torch.device("cuda")
-        poses = PoseList(plan, centers.detach().cpu().double().numpy(),
-                         axes.detach().cpu().double().numpy())
-        energy = torch.empty(len(centers), dtype=torch.float64, device=device)
+        c = centers.detach().to(device=device, dtype=torch.float64).contiguous()
+        u = axes.detach().to(device=device, dtype=torch.float64).contiguous()
+        energy = torch.empty(len(c), dtype=torch.float64, device=device)
+        grid = _grid_for(plan, len(c))
         torch.cuda.current_stream(device).synchronize()
-        poses.eval(precision, energy=energy.data_ptr())
-        ctx.poses = poses
+        grid.eval(len(c), c.data_ptr(), u.data_ptr(), precision, energy=energy.data_ptr())
+        ctx.save_for_backward(c, u)
+        ctx.grid = grid
         ctx.precision = precision
-        ctx.device = device
         ctx.dtypes = (centers.dtype, axes.dtype)
         return energy
 
     @staticmethod
     def backward(ctx, grad_energy):
-        n = ctx.poses.n
-        g = grad_energy.detach().to(device=ctx.device, dtype=torch.float64).contiguous()
-        grad_c = torch.empty((n, 3), dtype=torch.float64, device=ctx.device)
-        grad_u = torch.empty((n, 3), dtype=torch.float64, device=ctx.device)
-        torch.cuda.current_stream(ctx.device).synchronize()
-        ctx.poses.eval(ctx.precision, grad_out=g.data_ptr(), grad_center=grad_c.data_ptr(),
-                       grad_axis=grad_u.data_ptr())
+        c, u = ctx.saved_tensors
+        n = len(c)
+        g = grad_energy.detach().to(device=c.device, dtype=torch.float64).contiguous()
+        grad_c = torch.empty((n, 3), dtype=torch.float64, device=c.device)
+        grad_u = torch.empty((n, 3), dtype=torch.float64, device=c.device)
+        torch.cuda.current_stream(c.device).synchronize()
+        ctx.grid.eval(n, c.data_ptr(), u.data_ptr(), ctx.precision, grad_out=g.data_ptr(),
+                      grad_center=grad_c.data_ptr(), grad_axis=grad_u.data_ptr())
         return grad_c.to(ctx.dtypes[0]), grad_u.to(ctx.dtypes[1]), None, None
 
 

@@ -4,7 +4,8 @@ The tunables are
   * ``tile_centers`` -- pose centers per spatial tile.  Larger tiles mean
     fewer candidate lists but longer ones (each center streams and filters its
     tile's whole list); smaller tiles the opposite, plus more list-building.
-  * ``blocks_per_cu`` -- size of the persistent grid of a ragged batch.
+  * ``blocks_per_cu`` -- size of the persistent grid of a ragged batch (0: the
+    plain grid of one block per 4 work items, dispatched by the hardware).
 Candidates are timed on a reduced grid of a sample of the workload's own
 frameworks (HIP events on the batch stream: list build + field kernel), the
 fastest wins, and results are cached per (device, density class) in
@@ -22,7 +23,7 @@ from .field import Batch, Plan
 from .orientation import regular_pose_grid
 
 TILE_CANDIDATES = (16, 32, 64, 128, 256)
-BLOCK_CANDIDATES = (4, 8, 16)
+BLOCK_CANDIDATES = (0, 4, 5, 8)     # 0: one block per 4 work items, no queue
 
 
 def cache_path() -> Path:

@@ -95,6 +95,11 @@ struct Tile {
   int pad;
 };
 
+// A 3x3 matrix passed by value to kernels (row-major).
+struct Mat3 {
+  double m[9];
+};
+
 // Site-field work chunk: up to 64 member centers of one tile.
 struct SiteChunk {
   int job;
@@ -123,9 +128,21 @@ hipError_t launch_census(const Job* d_jobs, int n_jobs, const long long* d_job_w
                          hipStream_t s);
 hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job_first_center,
                            long long n_centers_total, hipStream_t s);
+// n_chunks bounds the grid; d_n_chunks (device, may be null) holds the count.
 hipError_t launch_pose(int precision, bool grad, const Job* d_jobs, const SiteChunk* d_chunks,
-                       int n_chunks, const long long* d_offsets, const float4* d_list32,
-                       const double* d_list64, hipStream_t s);
+                       int n_chunks, const int* d_n_chunks, const long long* d_offsets,
+                       const float4* d_list32, const double* d_list64, hipStream_t s);
+hipError_t launch_pose_bin(int n, const double* d_centers, const double* d_axes, const Mat3& Hinv,
+                           const int m[3], int* d_bin, int* d_rank, int* d_counts, int* d_bad,
+                           hipStream_t s);
+hipError_t launch_pose_tile_scan(const int* d_counts, int n_bins, int job, int* d_first,
+                                 SiteChunk* d_chunks, int* d_n_chunks, hipStream_t s);
+hipError_t launch_pose_scatter(int n, const double* d_centers, const double* d_axes, const Mat3& H,
+                               const Mat3& Hinv, const int m[3], const int* d_bin,
+                               const int* d_rank, const int* d_first, double* d_center_rel,
+                               int* d_center_id, int* d_tile_of, double* d_axes64, float* d_axes32,
+                               hipStream_t s);
+hipError_t launch_zero(int* p, int n, hipStream_t s);
 hipError_t launch_legacy(int n_centers, int n_axes, double cutoff2, const double* centers,
                          const double* axes, const double* images, const double* offsets,
                          const double* coeff12, const double* coeff6, const double* shift,

@@ -47,13 +47,18 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Blocks are dispatched round-robin over the 8 XCDs.  Map hardware block b to
-// a logical block so that consecutive logical blocks (same tile, same list)
-// run on the same XCD and share its L2.
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8).
+// Map hardware block b to a logical block so that runs of kXcdRun consecutive
+// logical blocks (same tile, same candidate list) execute on one XCD and share
+// its L2, while the runs rotate over the XCDs: every XCD gets an even share of
+// each part of the work range, so batches sorted by cost stay balanced.  The
+// last incomplete round maps to itself.
+constexpr long long kXcdRun = 16;
 __device__ __forceinline__ long long xcd_block(long long b, long long nb) {
-  const long long q = nb / 8, r = nb % 8;
+  const long long round = 8 * kXcdRun;
+  if (b >= nb / round * round) return b;
   const long long x = b % 8, i = b / 8;
-  return x * q + (x < r ? x : r) + i;
+  return ((i / kXcdRun) * 8 + x) * kXcdRun + i % kXcdRun;
 }
 
 __device__ __forceinline__ int find_job(const long long* __restrict__ job_work, int n_jobs,
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void f
 
 // Persistent form: a fixed grid whose waves drain the XCD-partitioned queue.
 template <bool SYM>
-__global__ __launch_bounds__(256) void field_mixed_persistent(const Job* __restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void field_mixed_persistent(const Job* __restrict__ jobs,
                                                               int n_jobs,
                                                               const long long* __restrict__ job_work,
                                                               long long n_work,
@@ -625,11 +630,15 @@ __device__ __forceinline__ void pose_pair64(double o, double ux, double uy, doub
 template <bool MIXED, bool GRAD>
 __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
                                                   const SiteChunk* __restrict__ chunks,
+                                                  const int* __restrict__ n_chunks,
                                                   const long long* __restrict__ offsets,
                                                   const float4* __restrict__ list32,
                                                   const double* __restrict__ list64) {
   __shared__ float4 s_e[64];
   __shared__ double s_d[3][64];
+  // A chunk table built on the device is launched with a grid that bounds its
+  // length; blocks past the actual count have nothing to do.
+  if (n_chunks && (int)blockIdx.x >= *n_chunks) return;
   const SiteChunk ch = chunks[blockIdx.x];
   const Job& J = jobs[ch.job];
   const int lane = threadIdx.x;
@@ -747,6 +756,125 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
       if (!isfinite(gc) || !isfinite(gu)) atomicOr(J.bad, 1);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Device binning of independent poses into a fixed grid of m0 x m1 x m2 tiles
+// (bins of the fractional cell).  Poses stay in device memory: bin + rank by
+// one atomic per pose, an exclusive scan over the bins that also cuts every
+// bin into <= 64-pose chunks, and a scatter into tile-sorted slots.  The
+// order of poses inside a bin depends on atomic timing, which changes no
+// result: each pose is evaluated by its own lane over its bin's list.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wrap01_dev(double x) {
+  double r = fmod(x, 1.0);
+  if (r != 0.0 && r < 0.0) r += 1.0;
+  return r;
+}
+
+__global__ __launch_bounds__(256) void pose_bin_kernel(int n, const double* __restrict__ centers,
+                                                       const double* __restrict__ axes, Mat3 Hinv,
+                                                       int m0, int m1, int m2,
+                                                       int* __restrict__ bin,
+                                                       int* __restrict__ rank,
+                                                       int* __restrict__ counts,
+                                                       int* __restrict__ bad) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double c[3] = {centers[3 * i], centers[3 * i + 1], centers[3 * i + 2]};
+  const double u0 = axes[3 * i], u1 = axes[3 * i + 1], u2 = axes[3 * i + 2];
+  const double nn = u0 * u0 + u1 * u1 + u2 * u2;
+  bool ok = isfinite(c[0]) && isfinite(c[1]) && isfinite(c[2]) && isfinite(nn) && nn > 0.0;
+  const int m[3] = {m0, m1, m2};
+  double f[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f[k] = wrap01_dev(c[0] * Hinv.m[k] + c[1] * Hinv.m[3 + k] + c[2] * Hinv.m[6 + k]);
+    ok = ok && f[k] >= 0.0 && f[k] <= 1.0;   // false for an overflowed (nonfinite) product
+  }
+  int idx[3] = {0, 0, 0};
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int q = (int)(f[k] * m[k]);
+      idx[k] = q < m[k] - 1 ? q : m[k] - 1;
+    }
+  } else {
+    atomicOr(bad, 2);
+  }
+  const int b = (idx[0] * m1 + idx[1]) * m2 + idx[2];
+  bin[i] = b;
+  rank[i] = atomicAdd(&counts[b], 1);
+}
+
+// One block: exclusive scan of the bin counts into first[] and the chunk
+// table (job, bin, first slot, <= 64 poses), whose length goes to *n_chunks.
+__global__ __launch_bounds__(1024) void pose_tile_scan_kernel(const int* __restrict__ counts,
+                                                              int n_bins, int job,
+                                                              int* __restrict__ first,
+                                                              SiteChunk* __restrict__ chunks,
+                                                              int* __restrict__ n_chunks) {
+  __shared__ int sa[1024], sb[1024];
+  const int t = threadIdx.x;
+  int carry_a = 0, carry_b = 0;
+  for (int base = 0; base < n_bins; base += 1024) {
+    const int i = base + t;
+    const int c = i < n_bins ? counts[i] : 0;
+    const int k = (c + 63) >> 6;
+    sa[t] = c;
+    sb[t] = k;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int va = t >= off ? sa[t - off] : 0, vb = t >= off ? sb[t - off] : 0;
+      __syncthreads();
+      sa[t] += va;
+      sb[t] += vb;
+      __syncthreads();
+    }
+    const int ea = carry_a + sa[t] - c, eb = carry_b + sb[t] - k;
+    if (i < n_bins) {
+      first[i] = ea;
+      for (int q = 0; q < k; ++q) {
+        const int left = c - 64 * q;
+        chunks[eb + q] = SiteChunk{job, i, ea + 64 * q, left < 64 ? left : 64};
+      }
+    }
+    carry_a += sa[1023];
+    carry_b += sb[1023];
+    __syncthreads();
+  }
+  if (t == 0) *n_chunks = carry_b;
+}
+
+// Tile-sorted slots: Cartesian offset from the bin centre (from fractional
+// differences, as for host-built pose sets), caller index, bin, director.
+__global__ __launch_bounds__(256) void pose_scatter_kernel(
+    int n, const double* __restrict__ centers, const double* __restrict__ axes, Mat3 H, Mat3 Hinv,
+    int m0, int m1, int m2, const int* __restrict__ bin, const int* __restrict__ rank,
+    const int* __restrict__ first, double* __restrict__ center_rel, int* __restrict__ center_id,
+    int* __restrict__ tile_of, double* __restrict__ axes64, float* __restrict__ axes32) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int b = bin[i];
+  const int slot = first[b] + rank[i];
+  const int idx[3] = {b / (m1 * m2), (b / m2) % m1, b % m2};
+  const int m[3] = {m0, m1, m2};
+  const double c[3] = {centers[3 * i], centers[3 * i + 1], centers[3 * i + 2]};
+  double d[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double f = wrap01_dev(c[0] * Hinv.m[k] + c[1] * Hinv.m[3 + k] + c[2] * Hinv.m[6 + k]);
+    d[k] = f - (idx[k] + 0.5) / m[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    center_rel[3 * (long long)slot + k] = d[0] * H.m[k] + d[1] * H.m[3 + k] + d[2] * H.m[6 + k];
+    const double u = axes[3 * i + k];
+    axes64[3 * (long long)slot + k] = u;
+    axes32[3 * (long long)slot + k] = (float)u;
+  }
+  center_id[slot] = i;
+  tile_of[slot] = b;
 }
 
 // Exact count of in-cutoff (site, image) pairs: the pair-eval census that the
@@ -984,21 +1112,53 @@ hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job
 }
 
 hipError_t launch_pose(int precision, bool grad, const Job* d_jobs, const SiteChunk* d_chunks,
-                       int n_chunks, const long long* d_offsets, const float4* d_list32,
-                       const double* d_list64, hipStream_t s) {
+                       int n_chunks, const int* d_n_chunks, const long long* d_offsets,
+                       const float4* d_list32, const double* d_list64, hipStream_t s) {
   if (n_chunks <= 0) return hipSuccess;
   const dim3 g(n_chunks), b(64);
   if (precision == PREC_MIXED) {
     if (grad)
-      hipLaunchKernelGGL((pose_kernel<true, true>), g, b, 0, s, d_jobs, d_chunks, d_offsets, d_list32, d_list64);
+      hipLaunchKernelGGL((pose_kernel<true, true>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,
+                         d_offsets, d_list32, d_list64);
     else
-      hipLaunchKernelGGL((pose_kernel<true, false>), g, b, 0, s, d_jobs, d_chunks, d_offsets, d_list32, d_list64);
+      hipLaunchKernelGGL((pose_kernel<true, false>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,
+                         d_offsets, d_list32, d_list64);
   } else {
     if (grad)
-      hipLaunchKernelGGL((pose_kernel<false, true>), g, b, 0, s, d_jobs, d_chunks, d_offsets, d_list32, d_list64);
+      hipLaunchKernelGGL((pose_kernel<false, true>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,
+                         d_offsets, d_list32, d_list64);
     else
-      hipLaunchKernelGGL((pose_kernel<false, false>), g, b, 0, s, d_jobs, d_chunks, d_offsets, d_list32, d_list64);
+      hipLaunchKernelGGL((pose_kernel<false, false>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,
+                         d_offsets, d_list32, d_list64);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_pose_bin(int n, const double* d_centers, const double* d_axes, const Mat3& Hinv,
+                           const int m[3], int* d_bin, int* d_rank, int* d_counts, int* d_bad,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pose_bin_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                     d_centers, d_axes, Hinv, m[0], m[1], m[2], d_bin, d_rank, d_counts, d_bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_pose_tile_scan(const int* d_counts, int n_bins, int job, int* d_first,
+                                 SiteChunk* d_chunks, int* d_n_chunks, hipStream_t s) {
+  hipLaunchKernelGGL(pose_tile_scan_kernel, dim3(1), dim3(1024), 0, s, d_counts, n_bins, job,
+                     d_first, d_chunks, d_n_chunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_pose_scatter(int n, const double* d_centers, const double* d_axes, const Mat3& H,
+                               const Mat3& Hinv, const int m[3], const int* d_bin,
+                               const int* d_rank, const int* d_first, double* d_center_rel,
+                               int* d_center_id, int* d_tile_of, double* d_axes64, float* d_axes32,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pose_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                     d_centers, d_axes, H, Hinv, m[0], m[1], m[2], d_bin, d_rank, d_first,
+                     d_center_rel, d_center_id, d_tile_of, d_axes64, d_axes32);
   return hipGetLastError();
 }
 

@@ -110,6 +110,24 @@ struct Batch {
 };
 
 
+// Device-resident pose grid: a fixed tile grid over the cell whose candidate
+// lists are built once; every evaluation bins the caller's device poses into
+// it on the GPU (no host copy of the poses).
+struct PoseGrid {
+  Plan* P = nullptr;
+  Poses* S = nullptr;        // the tile set (one tile per bin) and the slot arrays
+  Batch* B = nullptr;        // job table, candidate lists, stream
+  int m[3] = {1, 1, 1};
+  int n_bins = 0;
+  int capacity = 0;          // poses the slot arrays hold
+  Mat3 H{}, Hinv{};
+  DevBuf<int> bin, rank, counts, first, n_chunks;
+  ~PoseGrid() {
+    delete B;
+    delete S;
+  }
+};
+
 // Error reporting (thread-local message, C-ABI return code).
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);

@@ -542,7 +542,7 @@ extern "C" {
 
 const char* sv_last_error(void) { return t_error.c_str(); }
 
-int sv_abi_version(void) { return 1; }
+int sv_abi_version(void) { return 2; }
 
 int sv_device_count(void) {
   int n = 0;
@@ -908,12 +908,150 @@ int sv_pose_list_eval(void* poses, int precision, const double* grad_out, double
   }
   SV_HIP(launch_zero(B->d_bad.p, 1, B->stream));
   const bool grad = grad_center || grad_axis;
-  SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)B->chunks.size(),
+  SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)B->chunks.size(), nullptr,
                      B->d_offsets.p, B->d_list32.p, B->d_list64.p, B->stream));
   int bad = 0;
   SV_HIP(hipMemcpyAsync(&bad, B->d_bad.p, sizeof(int), hipMemcpyDeviceToHost, B->stream));
   SV_HIP(hipStreamSynchronize(B->stream));
   if (bad) return fail(4, "nonfinite pose energy or gradient");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident pose grids (PoseGrid in sv_objects.h).
+// ---------------------------------------------------------------------------
+int sv_pose_grid_create(void* plan, int n_expected, int tile_centers, void** grid) {
+  if (!plan || !grid) return fail(1, "null argument");
+  if (n_expected < 1) return fail(1, "n_expected must be >= 1");
+  if (tile_centers == 0) tile_centers = 64;
+  if (tile_centers < 1 || tile_centers > 1 << 20) return fail(1, "tile_centers must be in 1..2^20");
+  Plan* P = static_cast<Plan*>(plan);
+  SV_HIP(hipSetDevice(P->device));
+  auto G = new PoseGrid();
+  G->P = P;
+  choose_tiles(*P, n_expected, 0, tile_centers, G->m);
+  const long long nb = (long long)G->m[0] * G->m[1] * G->m[2];
+  if (nb > (1 << 22)) {
+    delete G;
+    return fail(1, "pose grid would need more than 2^22 tiles");
+  }
+  G->n_bins = (int)nb;
+  std::memcpy(G->H.m, P->H, sizeof(G->H.m));
+  std::memcpy(G->Hinv.m, P->Hinv, sizeof(G->Hinv.m));
+  // Every bin is a tile; its list radius covers the whole bin (the
+  // circumscribed sphere of the sub-cell a/m0, b/m1, c/m2 about its centre).
+  double radius = 0.0;
+  for (int sb = -1; sb <= 1; sb += 2)
+    for (int sc = -1; sc <= 1; sc += 2) {
+      double r2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double x = P->H[k] / G->m[0] + sb * P->H[3 + k] / G->m[1] + sc * P->H[6 + k] / G->m[2];
+        r2 += x * x;
+      }
+      radius = std::max(radius, 0.5 * std::sqrt(r2));
+    }
+  radius *= 1.0 + 1e-12;
+  auto S = new Poses();
+  G->S = S;
+  S->plan = P;
+  S->n_axes = 1;
+  S->n_groups = 1;
+  S->per_pose = true;
+  S->tiles.resize(G->n_bins);
+  for (int b = 0; b < G->n_bins; ++b) {
+    Tile& T = S->tiles[b];
+    T = Tile{};
+    const int idx[3] = {b / (G->m[1] * G->m[2]), (b / G->m[2]) % G->m[1], b % G->m[2]};
+    for (int k = 0; k < 3; ++k) T.frac[k] = (idx[k] + 0.5) / G->m[k];
+    T.radius = radius;
+    T.list_r = radius + P->keep_r + 1e-3;
+  }
+  Plan* pl = P;
+  Poses* po = S;
+  int rc = batch_build(1, &pl, &po, PREC_FP64, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, &G->B);
+  if (!rc) rc = batch_lists(G->B, false);
+  hipError_t e;
+  if (!rc && ((e = G->counts.reserve(G->n_bins)) != hipSuccess ||
+              (e = G->first.reserve(G->n_bins)) != hipSuccess ||
+              (e = G->n_chunks.reserve(1)) != hipSuccess))
+    rc = hip_fail(e, "pose grid buffers");
+  if (rc) {
+    delete G;
+    return rc;
+  }
+  *grid = G;
+  return 0;
+}
+
+int sv_pose_grid_eval(void* grid, int n, const double* centers_cart, const double* axes,
+                      int precision, const double* grad_out, double* energy, double* grad_center,
+                      double* grad_axis) {
+  if (!grid) return fail(1, "null pose grid");
+  if (n < 1) return fail(1, "at least one pose required");
+  if (!centers_cart || !axes) return fail(1, "null pose arrays");
+  if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
+  PoseGrid* G = static_cast<PoseGrid*>(grid);
+  Batch* B = G->B;
+  Poses* S = G->S;
+  SV_HIP(hipSetDevice(G->P->device));
+  const long long max_chunks = (long long)G->n_bins + (n + 63) / 64;
+  if (max_chunks > INT32_MAX) return fail(1, "too many poses");
+  if (n > G->capacity) {
+    const size_t c = (size_t)n + (size_t)n / 4;
+    SV_HIP(S->d_center_rel.reserve(3 * c));
+    SV_HIP(S->d_center_id.reserve(c));
+    SV_HIP(S->d_tile_of.reserve(c));
+    SV_HIP(S->d_pose_axes64.reserve(3 * c));
+    SV_HIP(S->d_pose_axes32.reserve(3 * c));
+    SV_HIP(G->bin.reserve(c));
+    SV_HIP(G->rank.reserve(c));
+    G->capacity = (int)std::min<size_t>(c, INT32_MAX);
+  }
+  SV_HIP(B->d_chunks.reserve((size_t)max_chunks, true));
+  Job& J = B->jobs[0];
+  J.center_rel = S->d_center_rel.p;
+  J.center_id = S->d_center_id.p;
+  J.tile_of = S->d_tile_of.p;
+  J.pose_axes64 = S->d_pose_axes64.p;
+  J.pose_axes32 = S->d_pose_axes32.p;
+  J.n_centers = n;
+  J.grad_out = grad_out;
+  J.pose_e = energy;
+  J.grad_c = grad_center;
+  J.grad_u = grad_axis;
+  hipStream_t st = B->stream;
+  SV_HIP(hipMemcpyAsync(B->d_jobs.p, &J, sizeof(Job), hipMemcpyHostToDevice, st));
+  SV_HIP(launch_zero(G->counts.p, G->n_bins, st));
+  SV_HIP(launch_zero(B->d_bad.p, 1, st));
+  SV_HIP(launch_pose_bin(n, centers_cart, axes, G->Hinv, G->m, G->bin.p, G->rank.p, G->counts.p,
+                         B->d_bad.p, st));
+  SV_HIP(launch_pose_tile_scan(G->counts.p, G->n_bins, 0, G->first.p, B->d_chunks.p,
+                               G->n_chunks.p, st));
+  SV_HIP(launch_pose_scatter(n, centers_cart, axes, G->H, G->Hinv, G->m, G->bin.p, G->rank.p,
+                             G->first.p, S->d_center_rel.p, S->d_center_id.p, S->d_tile_of.p,
+                             S->d_pose_axes64.p, S->d_pose_axes32.p, st));
+  const bool grad = grad_center || grad_axis;
+  SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)max_chunks, G->n_chunks.p,
+                     B->d_offsets.p, B->d_list32.p, B->d_list64.p, st));
+  int bad = 0;
+  SV_HIP(hipMemcpyAsync(&bad, B->d_bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  SV_HIP(hipStreamSynchronize(st));
+  if (bad & 2) return fail(1, "nonfinite pose center or director, or zero director");
+  if (bad) return fail(4, "nonfinite pose energy or gradient");
+  return 0;
+}
+
+int sv_pose_grid_info(void* grid, int* n_tiles, int* capacity) {
+  if (!grid) return fail(1, "null pose grid");
+  PoseGrid* G = static_cast<PoseGrid*>(grid);
+  if (n_tiles) *n_tiles = G->n_bins;
+  if (capacity) *capacity = G->capacity;
+  return 0;
+}
+
+int sv_pose_grid_destroy(void* grid) {
+  delete static_cast<PoseGrid*>(grid);
   return 0;
 }
 

@@ -278,6 +278,45 @@ class Batch:
         self.close()
 
 
+class PoseGrid:
+    """Device-resident independent poses (sv_pose_grid_*).
+
+    A fixed tile grid over the cell, sized for about ``n_expected`` poses, whose
+    candidate lists are built once.  ``eval`` takes device pointers (e.g. torch
+    ``data_ptr()``) of FP64 ``centers_cart`` and ``axes`` arrays of shape
+    (n, 3), bins them into the tiles on the GPU and writes the requested
+    device outputs; the poses never visit the host.
+    """
+
+    def __init__(self, plan: Plan, n_expected: int, tile_centers: int = 64):
+        self.lib = plan.lib
+        self.plan = plan
+        handle = ctypes.c_void_p()
+        _lib.check(self.lib.sv_pose_grid_create(plan.handle, int(n_expected), int(tile_centers),
+                                                ctypes.byref(handle)))
+        self.handle = handle.value
+
+    def eval(self, n, centers, axes, precision="fp64", grad_out=None, energy=None,
+             grad_center=None, grad_axis=None):
+        """All arguments after ``n`` are device pointers (ints) or None."""
+        _lib.check(self.lib.sv_pose_grid_eval(self.handle, int(n), centers, axes,
+                                              precision_code(precision), grad_out, energy,
+                                              grad_center, grad_axis))
+
+    def info(self):
+        tiles, cap = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self.lib.sv_pose_grid_info(self.handle, ctypes.byref(tiles), ctypes.byref(cap)))
+        return {"tiles": tiles.value, "capacity": cap.value}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.sv_pose_grid_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
 class PoseList:
     """Independent poses (center_i, director_i) on the device (sv_pose_list_create).
 

@@ -168,3 +168,66 @@ def test_autograd_function(oracle, case, poses):
     assert torch.isfinite(v.grad).all()
     radial = (v.grad * v.detach()).sum(dim=1).abs().max().item()
     assert radial < 1e-6 * max(1.0, v.grad.abs().max().item())
+
+
+def test_pose_grid_device_binning_matches_oracle_and_host_binning(oracle, case, poses):
+    """sv_pose_grid_eval: poses binned on the GPU from device arrays, against the
+    FP64 oracle (energy and gradient) and against the host-binned pose list."""
+    import torch
+    from string_vext_amd.field import PoseGrid, PoseList
+    centers, axes = poses
+    n = len(centers)
+    e_ref, gc_ref, gu_ref = oracle.pose_grad(case, centers, axes)
+    plan = _plan(case)
+    grid = PoseGrid(plan, n_expected=n)
+    c = torch.tensor(centers, dtype=torch.float64, device="cuda")
+    u = torch.tensor(axes, dtype=torch.float64, device="cuda")
+    for precision in ("fp64", "mixed"):
+        e = torch.empty(n, dtype=torch.float64, device="cuda")
+        gc = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+        gu = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+        grid.eval(n, c.data_ptr(), u.data_ptr(), precision, energy=e.data_ptr())
+        grid.eval(n, c.data_ptr(), u.data_ptr(), precision, grad_center=gc.data_ptr(),
+                  grad_axis=gu.data_ptr())
+        e, gc, gu = e.cpu().numpy(), gc.cpu().numpy(), gu.cpu().numpy()
+        if precision == "fp64":
+            assert (np.abs(e - e_ref) / np.maximum(np.abs(e_ref), 1.0)).max() < 1e-12
+            for g, ref in ((gc, gc_ref), (gu, gu_ref)):
+                scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)
+                assert (np.abs(g - ref) / scale).max() < 1e-10
+            e_host = PoseList(plan, centers, axes).energy_host("fp64")
+            assert (np.abs(e - e_host) / np.maximum(np.abs(e_host), 1.0)).max() < 1e-13
+        else:
+            m = oracle.error_metrics(e_ref, e)
+            assert m["accessible_max_abs_error_K"] < 1e-3
+            assert m["all_max_relative_error_floor1K"] < 1e-5
+            for g, ref in ((gc, gc_ref), (gu, gu_ref)):
+                scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)
+                assert (np.abs(g - ref) / scale).max() < 1e-4
+
+
+def test_pose_grid_grows_shrinks_and_rejects_nonfinite_poses(oracle, case, poses):
+    import torch
+    from string_vext_amd.field import PoseGrid
+    centers, axes = poses
+    plan = _plan(case)
+    grid = PoseGrid(plan, n_expected=64)
+    for n in (len(centers), 5, 300):          # grow past the initial capacity, then reuse
+        c = torch.tensor(centers[:n], dtype=torch.float64, device="cuda")
+        u = torch.tensor(axes[:n], dtype=torch.float64, device="cuda")
+        e = torch.empty(n, dtype=torch.float64, device="cuda")
+        grid.eval(n, c.data_ptr(), u.data_ptr(), "fp64", energy=e.data_ptr())
+        e_ref = oracle.pose_grad(case, centers[:n], axes[:n])[0]
+        assert (np.abs(e.cpu().numpy() - e_ref) / np.maximum(np.abs(e_ref), 1.0)).max() < 1e-12
+    assert grid.info()["capacity"] >= len(centers)
+    # a nonfinite center is flagged by the binning kernel and reported as code 1
+    bad = torch.tensor(centers[:8], dtype=torch.float64, device="cuda")
+    bad[3, 1] = float("nan")
+    u = torch.tensor(axes[:8], dtype=torch.float64, device="cuda")
+    e = torch.empty(8, dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError, match="code 1"):
+        grid.eval(8, bad.data_ptr(), u.data_ptr(), "mixed", energy=e.data_ptr())
+    # and the grid stays usable
+    c = torch.tensor(centers[:8], dtype=torch.float64, device="cuda")
+    grid.eval(8, c.data_ptr(), u.data_ptr(), "fp64", energy=e.data_ptr())
+    assert np.isfinite(e.cpu().numpy()).all()
