@@ -1,6 +1,7 @@
 """Pins the CPU restatement of one String iteration (oracle/string_oracle.py)
 by the invariants the reference's kernels guarantee."""
 import numpy as np
+import pytest
 
 from oracle import string_oracle as SO
 
@@ -52,3 +53,91 @@ def test_convergence_thresholds():
     n, mf, ma = 96, 1e-4, np.radians(1.0)
     assert np.isclose(0.30 * mf * n * np.sqrt(3), 0.004988, rtol=1e-3)
     assert np.isclose(0.30 * ma * n * np.sqrt(3), 0.870568, rtol=1e-3)
+
+
+def _v_string(case, x):
+    """Sum of the reference String pair (restated in oracle_string_pair) over both
+    guest sites and every framework atom, at pose x = (a, b, c, alpha, beta, gamma)."""
+    from oracle import vext_oracle as O
+    cell = np.asarray(case["cell"])
+    c = x[:3] @ cell
+    u = SO.director(x[4], x[5])
+    gs, ge = case["guest_sigma"][0], case["guest_epsilon"][0]
+    total = 0.0
+    for o in O.body_offsets(case):
+        site = c + o * u
+        for f, s, e in zip(case["frame_frac"], case["frame_sigma"], case["frame_epsilon"]):
+            total += O.string_pair(site, f, cell, ge, gs, e, s, case["cutoff"])
+    return total
+
+
+def test_energy_grad_equals_string_pair_sum_and_central_differences():
+    from string_vext_amd import synthetic
+    case = synthetic.small_field_case()
+    sub = dict(case)
+    keep = slice(0, 40)                    # 40 atoms keep the pure-Python pair sum quick
+    for key in ("frame_frac", "frame_sigma", "frame_epsilon"):
+        sub[key] = np.asarray(case[key])[keep]
+    rng = np.random.default_rng(11)
+    x = np.c_[rng.random((6, 3)), rng.uniform(0, 2 * np.pi, (6, 3))]
+    e, g = SO.energy_grad(sub, x)
+    for i in range(len(x)):
+        v = _v_string(sub, x[i])
+        assert e[i] == pytest.approx(v, rel=1e-10, abs=1e-8)
+        if abs(v) > 1e4:
+            continue                        # overlap poses: differences are not informative
+        for k in (0, 1, 2, 4, 5):
+            h = 1e-6 if k < 3 else 1e-6
+            xp, xm = x[i].copy(), x[i].copy()
+            xp[k] += h
+            xm[k] -= h
+            fd = (_v_string(sub, xp) - _v_string(sub, xm)) / (2 * h)
+            assert g[i, k] == pytest.approx(fd, rel=1e-5, abs=1e-5)
+        assert g[i, 3] == 0.0              # alpha rotates the guest about its own axis
+
+
+def test_revert_restores_angles_of_images_whose_energy_rose(monkeypatch):
+    n = 7
+    s0 = np.zeros((n, 6))
+    s0[:, 0] = np.linspace(0, 1, n)
+    s0[:, 1:3] = 0.5
+    s0[:, 3:] = np.linspace(1.0, 2.0, n)[:, None] * np.array([1.0, 0.5, 0.25])
+    grad = np.zeros((n, 6))
+    grad[:, 4:] = 30.0                      # moves beta and gamma of every image
+    calls = {"n": 0}
+
+    def fake(case_, x, setup=None):
+        calls["n"] += 1
+        v = np.zeros(len(x))
+        if calls["n"] == 2:                 # energies after the step: odd images rose
+            v[1::2] = 1.0
+        return v, grad.copy()
+
+    monkeypatch.setattr(SO, "energy_grad", fake)
+    out, conv = SO.iterate({"cell": np.diag([10.0, 10.0, 10.0])}, s0, 1, it=1)
+    assert not conv
+    # expected: stepped angles where the energy did not rise, old angles where it did
+    mv = np.radians(1.0)
+    g = SO._floor(mv * grad)
+    nr = SO._floor(np.sqrt((g[:, 3:] ** 2).sum(1)))
+    stepped = s0[:, 3:] - mv * np.where(nr[:, None] < mv, g[:, 3:], SO._floor(g[:, 3:] / nr[:, None]))
+    stepped[0] = s0[0, 3:] - mv * SO._floor(g[0, 3:] / nr[0])
+    stepped[-1] = stepped[0]                 # last image copies the first (periodic string)
+    ang = np.where((np.arange(n) % 2 == 1)[:, None], s0[:, 3:], stepped)
+    expect = SO.smooth(ang)
+    np.testing.assert_allclose(out[:, 3:], expect, rtol=0, atol=1e-15)
+    assert not np.allclose(out[1, 3:], stepped[1])
+
+
+def test_converged_string_stops_on_a_check_iteration(monkeypatch):
+    n = 9
+    s0 = np.zeros((n, 6))
+    s0[:, 0] = np.linspace(0, 1, n)
+    s0[:, 3:] = 1.0
+    monkeypatch.setattr(SO, "energy_grad",
+                        lambda case_, x, setup=None: (np.zeros(len(x)), np.zeros((len(x), 6))))
+    cell = np.diag([10.0, 10.0, 10.0])
+    out, conv = SO.iterate({"cell": cell}, s0, 1, it=200)
+    assert conv and np.array_equal(out, s0)
+    out, conv = SO.iterate({"cell": cell}, s0, 1, it=199)   # not a check iteration
+    assert not conv
