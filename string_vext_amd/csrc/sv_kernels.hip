@@ -371,8 +371,10 @@ __device__ __forceinline__ double pair64_d2(double d2, double rc2, double c12, d
 constexpr int kFlush = 16;
 
 struct MixedLds {
-  float4 in[4][64];
+  float4 in[4][64];    // (rx, ry, rz, |r|^2 + o0^2)
   float4 far[4][64];
+  float in1[4][64];    // |r|^2 + o1^2 (asymmetric guests only)
+  float far1[4][64];
   int near[4][64];
 };
 
@@ -398,7 +400,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
                cz = J.center_rel[3 * c.slot + 2];
   const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
-  const float o0sq = J.o0sqf, dR2 = J.dR2f, in2 = J.in2f;
+  const float o0sq = J.o0sqf, o1sq = (float)(J.o1 * J.o1), in2 = J.in2f;
   const double rc2 = J.rc2;
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
   const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1;
@@ -428,8 +430,16 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
       const unsigned long long min_ = __ballot(is_in);
       const unsigned long long mfar = __ballot(is_far);
       const unsigned long long mnear = __ballot(is_near);
-      if (is_in) s_in[wv][lanes_below(min_)] = make_float4(rx, ry, rz, rr + o0sq);
-      if (is_far) s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, rr + o0sq);
+      // Each site's |r|^2 + o_s^2 is rounded once, here; an asymmetric guest's
+      // second base is stored beside the entry rather than derived from the first.
+      if (is_in) {
+        s_in[wv][lanes_below(min_)] = make_float4(rx, ry, rz, rr + o0sq);
+        if (!SYM) L.in1[wv][lanes_below(min_)] = rr + o1sq;
+      }
+      if (is_far) {
+        s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, rr + o0sq);
+        if (!SYM) L.far1[wv][lanes_below(mfar)] = rr + o1sq;
+      }
       if (is_near) s_near[wv][lanes_below(mnear)] = (int)(idx - base);
       if (do_site) {
         // far candidates: clamped FP32 term (the shift is added per kept entry below);
@@ -459,7 +469,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
           const float4 f = s_in[wv][q];
           const float tdot = f.x * ux + f.y * uy + f.z * uz;
           const float d0 = fmaf(tdot, m2o0, f.w);
-          const float d1 = fmaf(tdot, m2o1, SYM ? f.w : f.w + dR2);
+          const float d1 = fmaf(tdot, m2o1, SYM ? f.w : L.in1[wv][q]);
           const float i0 = __builtin_amdgcn_rcpf(d0);
           const float i1 = __builtin_amdgcn_rcpf(d1);
           const float i06 = i0 * i0 * i0;
@@ -478,7 +488,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
           const float4 f = s_far[wv][q];
           const float tdot = f.x * ux + f.y * uy + f.z * uz;
           float d0 = fmaf(tdot, m2o0, f.w);
-          float d1 = fmaf(tdot, m2o1, SYM ? f.w : f.w + dR2);
+          float d1 = fmaf(tdot, m2o1, SYM ? f.w : L.far1[wv][q]);
           d0 = __builtin_amdgcn_fmed3f(d0, fl2f, rc2f);
           d1 = __builtin_amdgcn_fmed3f(d1, fl2f, rc2f);
           const float i0 = __builtin_amdgcn_rcpf(d0);
@@ -652,7 +662,7 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
   const float uxf = (float)ux, uyf = (float)uy, uzf = (float)uz;
   const double o0 = J.o0, o1 = J.o1, rc2 = J.rc2;
   const float rc2f = J.rc2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
-  const float o0sq = J.o0sqf, o1sq = J.o0sqf + J.dR2f;
+  const float o0sq = J.o0sqf, o1sq = (float)(J.o1 * J.o1);
   const float o0f = (float)o0, o1f = (float)o1;
   const long long seg0 = J.seg_base + (long long)ch.tile * J.n_types;
   PoseAcc a = {};
