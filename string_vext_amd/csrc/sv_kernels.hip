@@ -375,7 +375,7 @@ struct MixedLds {
   float4 far[4][64];
   float in1[4][64];    // |r|^2 + o1^2 (asymmetric guests only)
   float far1[4][64];
-  int near[4][64];
+  double4 near[4][64]; // FP64 (qx, qy, qz, |q|^2) of near-contact atoms
 };
 
 // One pose center x 64 orientations of the mixed-precision field.
@@ -386,7 +386,6 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
                                            const double* __restrict__ list64) {
   float4 (*s_in)[64] = L.in;
   float4 (*s_far)[64] = L.far;
-  int (*s_near)[64] = L.near;
   const int lane = threadIdx.x & 63;
   const Job& J = *c.J;
   // The site field of a center is the same pair sum taken at the center
@@ -440,7 +439,15 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, rr + o0sq);
         if (!SYM) L.far1[wv][lanes_below(mfar)] = rr + o1sq;
       }
-      if (is_near) s_near[wv][lanes_below(mnear)] = (int)(idx - base);
+      // Near-contact atoms: the compacting lane forms the FP64 center-relative
+      // vector once and stages it in LDS for every orientation lane.
+      double RRn = 0.0;
+      if (is_near) {
+        const double qx = list64[3 * idx] - cx, qy = list64[3 * idx + 1] - cy,
+                     qz = list64[3 * idx + 2] - cz;
+        RRn = qx * qx + qy * qy + qz * qz;
+        L.near[wv][lanes_below(mnear)] = make_double4(qx, qy, qz, RRn);
+      }
       if (do_site) {
         // far candidates: clamped FP32 term (the shift is added per kept entry below);
         // near candidates: the FP64 reference term from the FP64 offsets.
@@ -449,11 +456,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         const float is6 = is * is * is;
         const float sf = is6 * fmaf(is6, s6, -1.0f);
         S = fma(c6, (double)(is_far || is_in ? sf : 0.0f), S);
-        if (is_near) {
-          const double qx = list64[3 * idx] - cx, qy = list64[3 * idx + 1] - cy,
-                       qz = list64[3 * idx + 2] - cz;
-          S += pair64_d2(qx * qx + qy * qy + qz * qz, rc2, c12, c6, sh, fl2);
-        }
+        if (is_near) S += pair64_d2(RRn, rc2, c12, c6, sh, fl2);
       }
       wave_sync();
       const int ni = __popcll(min_), nf = __popcll(mfar), nn = __popcll(mnear);
@@ -503,13 +506,10 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
       }
       for (int q = 0; q < nn; ++q) {
         // FP64 near-contact pair: one dot product serves both sites.
-        const long long id = base + s_near[wv][q];
-        const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
-                     qz = list64[3 * id + 2] - cz;
-        const double RR = qx * qx + qy * qy + qz * qz;
-        const double tt = qx * ux64 + qy * uy64 + qz * uz64;
-        E += pair64_d2(fma(tt, m2o0d, RR + o0sqd), rc2, c12, c6, sh, fl2);
-        E += pair64_d2(fma(tt, m2o1d, RR + o1sqd), rc2, c12, c6, sh, fl2);
+        const double4 d = L.near[wv][q];
+        const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
+        E += pair64_d2(fma(tt, m2o0d, d.w + o0sqd), rc2, c12, c6, sh, fl2);
+        E += pair64_d2(fma(tt, m2o1d, d.w + o1sqd), rc2, c12, c6, sh, fl2);
       }
       wave_sync();
     }
