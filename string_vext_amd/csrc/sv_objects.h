@@ -120,6 +120,7 @@ struct PoseGrid {
   int m[3] = {1, 1, 1};
   int n_bins = 0;
   int capacity = 0;          // poses the slot arrays hold
+  double radius = 0.0;       // circumscribed radius of one bin
   Mat3 H{}, Hinv{};
   DevBuf<int> bin, rank, counts, first, n_chunks;
   ~PoseGrid() {
@@ -134,5 +135,8 @@ int hip_fail(hipError_t e, const char* what);
 
 // Framework and guest fields of a job from a plan (outputs and pose fields untouched).
 void plan_fill_job(const Plan& P, Job& J);
+
+// Pose grid of m[0] x m[1] x m[2] bins with its candidate lists built (sv_runtime.hip).
+int pose_grid_build(Plan* P, const int m[3], PoseGrid** out);
 
 }  // namespace sv

@@ -362,6 +362,71 @@ int batch_lists(Batch* B, bool timed) {
   return 0;
 }
 
+}  // namespace
+
+// A pose grid of m[0] x m[1] x m[2] bins over the cell.  Every bin is a tile
+// whose list radius covers the whole bin (the circumscribed sphere of the
+// sub-cell a/m0, b/m1, c/m2 about its centre); the lists are built here, once.
+int pose_grid_build(Plan* P, const int m_in[3], PoseGrid** out) {
+  for (int k = 0; k < 3; ++k)
+    if (m_in[k] < 1 || m_in[k] > 1024) return fail(1, "tile counts must be in 1..1024");
+  const long long nb = (long long)m_in[0] * m_in[1] * m_in[2];
+  if (nb > (1 << 22)) return fail(1, "pose grid would need more than 2^22 tiles");
+  hipError_t e;
+  if ((e = hipSetDevice(P->device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+  auto G = new PoseGrid();
+  G->P = P;
+  for (int k = 0; k < 3; ++k) G->m[k] = m_in[k];
+  G->n_bins = (int)nb;
+  std::memcpy(G->H.m, P->H, sizeof(G->H.m));
+  std::memcpy(G->Hinv.m, P->Hinv, sizeof(G->Hinv.m));
+  double radius = 0.0;
+  for (int sb = -1; sb <= 1; sb += 2)
+    for (int sc = -1; sc <= 1; sc += 2) {
+      double r2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double x = P->H[k] / G->m[0] + sb * P->H[3 + k] / G->m[1] + sc * P->H[6 + k] / G->m[2];
+        r2 += x * x;
+      }
+      radius = std::max(radius, 0.5 * std::sqrt(r2));
+    }
+  radius *= 1.0 + 1e-12;
+  G->radius = radius;
+  auto S = new Poses();
+  G->S = S;
+  S->plan = P;
+  S->n_axes = 1;
+  S->n_groups = 1;
+  S->per_pose = true;
+  S->tiles.resize(G->n_bins);
+  for (int b = 0; b < G->n_bins; ++b) {
+    Tile& T = S->tiles[b];
+    T = Tile{};
+    const int idx[3] = {b / (G->m[1] * G->m[2]), (b / G->m[2]) % G->m[1], b % G->m[2]};
+    for (int k = 0; k < 3; ++k) T.frac[k] = (idx[k] + 0.5) / G->m[k];
+    T.radius = radius;
+    T.list_r = radius + P->keep_r + 1e-3;
+  }
+  Plan* pl = P;
+  Poses* po = S;
+  int rc = batch_build(1, &pl, &po, PREC_FP64, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, &G->B);
+  if (!rc) rc = batch_lists(G->B, false);
+  if (!rc && ((e = G->counts.reserve(G->n_bins)) != hipSuccess ||
+              (e = G->first.reserve(G->n_bins)) != hipSuccess ||
+              (e = G->n_chunks.reserve(1)) != hipSuccess ||
+              (e = hipStreamSynchronize(G->B->stream)) != hipSuccess))
+    rc = hip_fail(e, "pose grid buffers");
+  if (rc) {
+    delete G;
+    return rc;
+  }
+  *out = G;
+  return 0;
+}
+
+namespace {
+
 int batch_run(Batch* B, bool timed, bool sync) {
   SV_HIP(hipSetDevice(B->device));
   int rc = batch_lists(B, timed);
@@ -926,60 +991,11 @@ int sv_pose_grid_create(void* plan, int n_expected, int tile_centers, void** gri
   if (tile_centers == 0) tile_centers = 64;
   if (tile_centers < 1 || tile_centers > 1 << 20) return fail(1, "tile_centers must be in 1..2^20");
   Plan* P = static_cast<Plan*>(plan);
-  SV_HIP(hipSetDevice(P->device));
-  auto G = new PoseGrid();
-  G->P = P;
-  choose_tiles(*P, n_expected, 0, tile_centers, G->m);
-  const long long nb = (long long)G->m[0] * G->m[1] * G->m[2];
-  if (nb > (1 << 22)) {
-    delete G;
-    return fail(1, "pose grid would need more than 2^22 tiles");
-  }
-  G->n_bins = (int)nb;
-  std::memcpy(G->H.m, P->H, sizeof(G->H.m));
-  std::memcpy(G->Hinv.m, P->Hinv, sizeof(G->Hinv.m));
-  // Every bin is a tile; its list radius covers the whole bin (the
-  // circumscribed sphere of the sub-cell a/m0, b/m1, c/m2 about its centre).
-  double radius = 0.0;
-  for (int sb = -1; sb <= 1; sb += 2)
-    for (int sc = -1; sc <= 1; sc += 2) {
-      double r2 = 0.0;
-      for (int k = 0; k < 3; ++k) {
-        const double x = P->H[k] / G->m[0] + sb * P->H[3 + k] / G->m[1] + sc * P->H[6 + k] / G->m[2];
-        r2 += x * x;
-      }
-      radius = std::max(radius, 0.5 * std::sqrt(r2));
-    }
-  radius *= 1.0 + 1e-12;
-  auto S = new Poses();
-  G->S = S;
-  S->plan = P;
-  S->n_axes = 1;
-  S->n_groups = 1;
-  S->per_pose = true;
-  S->tiles.resize(G->n_bins);
-  for (int b = 0; b < G->n_bins; ++b) {
-    Tile& T = S->tiles[b];
-    T = Tile{};
-    const int idx[3] = {b / (G->m[1] * G->m[2]), (b / G->m[2]) % G->m[1], b % G->m[2]};
-    for (int k = 0; k < 3; ++k) T.frac[k] = (idx[k] + 0.5) / G->m[k];
-    T.radius = radius;
-    T.list_r = radius + P->keep_r + 1e-3;
-  }
-  Plan* pl = P;
-  Poses* po = S;
-  int rc = batch_build(1, &pl, &po, PREC_FP64, nullptr, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, &G->B);
-  if (!rc) rc = batch_lists(G->B, false);
-  hipError_t e;
-  if (!rc && ((e = G->counts.reserve(G->n_bins)) != hipSuccess ||
-              (e = G->first.reserve(G->n_bins)) != hipSuccess ||
-              (e = G->n_chunks.reserve(1)) != hipSuccess))
-    rc = hip_fail(e, "pose grid buffers");
-  if (rc) {
-    delete G;
-    return rc;
-  }
+  int m[3];
+  choose_tiles(*P, n_expected, 0, tile_centers, m);
+  PoseGrid* G = nullptr;
+  const int rc = pose_grid_build(P, m, &G);
+  if (rc) return rc;
   *grid = G;
   return 0;
 }

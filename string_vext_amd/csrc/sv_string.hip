@@ -26,10 +26,11 @@
 // The director of the body-x guest is (cos g cos b, sin g cos b, -sin b)
 // (device_fxn.h:34-63), so alpha carries no force.
 //
-// Energies of string images are evaluated one wave per image, lanes over
-// framework atoms and their lattice images inside rc + max|o| (no candidate
-// list: images move every iteration).  One iteration is four kernels; the
-// runtime captures them in a HIP graph and replays it.
+// Energies of string images are evaluated one wave per image: the image is
+// located in a fixed grid of ~2 A bins over the cell whose candidate lists
+// (tile-relative, FP32 + FP64) are built once per framework, and the lanes
+// stride over its bin's list.  One iteration is four kernels; the runtime
+// captures them in a HIP graph and replays it.
 #include "sv_objects.h"
 #include "sv_vext.h"
 
@@ -45,6 +46,7 @@ constexpr double kPi = 3.141592653589793;
 constexpr double kRound = 1e-15;   // RC_main.cu:1733
 constexpr double kSmooth = 1e-4;   // RC_main.cu:1734
 constexpr int kCheckEvery = 200;   // RC_main.cu:2361
+constexpr double kBinEdge = 2.0;   // A, bin size of the string images' candidate lists
 
 struct StringParams {
   int n_strings;
@@ -65,10 +67,30 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 
 __device__ __forceinline__ double grid_floor(double v) { return floor(v / kRound) * kRound; }
 
+// Candidate lists of one framework for string images: a fixed grid of bins
+// over the cell (bins of about 2 A), each with the tile-relative list of every
+// image within (bin radius + rc + max|o|) of its centre, built once.
+struct ImageGrid {
+  const Job* job;            // coefficients, cell, list layout (segment = bin * n_types + type)
+  double Hinv[9];
+  int m[3];
+  int pad;
+  const long long* offsets;
+  const float4* list32;
+  const double* list64;
+};
+
+__device__ __forceinline__ double wrap01_s(double x) {
+  double r = fmod(x, 1.0);
+  if (r != 0.0 && r < 0.0) r += 1.0;
+  return r;
+}
+
 // Energy (and gradient) of one pose per wave.  x = (a, b, c, alpha, beta,
-// gamma) with fractional center (a, b, c).  G receives dV/dx.
+// gamma) with fractional center (a, b, c).  The image is located in its bin;
+// lanes stride over the bin's candidate list.  G receives dV/dx.
 template <bool MIXED, bool GRAD>
-__global__ __launch_bounds__(256) void image_kernel(const Job* __restrict__ plans,
+__global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict__ grids,
                                                     const int* __restrict__ plan_of, int n_images,
                                                     int n_poses, const int* __restrict__ converged,
                                                     const double* __restrict__ x,
@@ -78,79 +100,91 @@ __global__ __launch_bounds__(256) void image_kernel(const Job* __restrict__ plan
   if (p >= n_poses) return;
   const int s = p / n_images;
   if (converged && converged[s]) return;
-  const Job& J = plans[plan_of[s]];
-  const double f[3] = {x[6 * p], x[6 * p + 1], x[6 * p + 2]};
+  const ImageGrid& IG = grids[plan_of[s]];
+  const Job& J = *IG.job;
   const double be = x[6 * p + 4], ga = x[6 * p + 5];
   const double u[3] = {cos(ga) * cos(be), sin(ga) * cos(be), -sin(be)};
   const float uf[3] = {(float)u[0], (float)u[1], (float)u[2]};
+  // bin of the center and its offset from the bin centre (fractional differences)
+  int idx[3];
+  double df[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double f = wrap01_s(x[6 * p + k]);
+    if (!(f >= 0.0 && f <= 1.0)) f = 0.0;   // nonfinite input: stay inside the grid
+    const int q = (int)(f * IG.m[k]);
+    idx[k] = q < IG.m[k] - 1 ? q : IG.m[k] - 1;
+    df[k] = f - (idx[k] + 0.5) / IG.m[k];
+  }
+  double c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[k] = df[0] * J.H[k] + df[1] * J.H[3 + k] + df[2] * J.H[6 + k];
+  const float cf[3] = {(float)c[0], (float)c[1], (float)c[2]};
+  const long long seg0 = (long long)((idx[0] * IG.m[1] + idx[1]) * IG.m[2] + idx[2]) * J.n_types;
   const double o[2] = {J.o0, J.o1};
-  const double keep_r = J.keep_r, keep2 = keep_r * keep_r, rc2 = J.rc2;
+  const double keep2 = J.keep_r * J.keep_r, rc2 = J.rc2;
   double E = 0.0, gc[3] = {0.0, 0.0, 0.0}, gu[3] = {0.0, 0.0, 0.0};
   for (int t = 0; t < J.n_types; ++t) {
     const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
-    for (int a = J.type_start[t] + lane; a < J.type_start[t + 1]; a += 64) {
-      double d[3];
-      for (int k = 0; k < 3; ++k) d[k] = J.frac[3 * a + k] - f[k];
-      int lo[3], hi[3];
-      for (int k = 0; k < 3; ++k) {
-        const double h = keep_r * J.hn[k];
-        lo[k] = (int)ceil(-d[k] - h);
-        hi[k] = (int)floor(-d[k] + h);
+    const long long beg = IG.offsets[seg0 + t], end = IG.offsets[seg0 + t + 1];
+    for (long long i = beg + lane; i < end; i += 64) {
+      bool fp64 = !MIXED;
+      double r[3];
+      float rf[3], rrf = 0.0f;
+      if (MIXED) {
+        const float4 e = IG.list32[i];
+        rf[0] = e.x - cf[0];
+        rf[1] = e.y - cf[1];
+        rf[2] = e.z - cf[2];
+        rrf = rf[0] * rf[0] + rf[1] * rf[1] + rf[2] * rf[2];
+        if (!(rrf < J.keep2f)) continue;
+        fp64 = rrf < near2;
       }
-      for (int na = lo[0]; na <= hi[0]; ++na)
-        for (int nb = lo[1]; nb <= hi[1]; ++nb)
-          for (int nc = lo[2]; nc <= hi[2]; ++nc) {
-            const double ua = d[0] + na, ub = d[1] + nb, uc = d[2] + nc;
-            const double r[3] = {ua * J.H[0] + ub * J.H[3] + uc * J.H[6],
-                                 ua * J.H[1] + ub * J.H[4] + uc * J.H[7],
-                                 ua * J.H[2] + ub * J.H[5] + uc * J.H[8]};
-            const double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-            if (!(rr < keep2)) continue;
-            if (!MIXED || rr < (double)near2) {
-              for (int si = 0; si < 2; ++si) {
-                const double dv[3] = {o[si] * u[0] - r[0], o[si] * u[1] - r[1], o[si] * u[2] - r[2]};
-                const double d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
-                if (d2 < rc2) {
-                  const bool floored = d2 < fl2;
-                  const double inv = 1.0 / (floored ? fl2 : d2);
-                  const double i6 = inv * inv * inv;
-                  E += c12 * i6 * i6 - c6 * i6 + sh;
-                  if (GRAD && !floored) {
-                    const double g = 2.0 * (-6.0 * c12 * i6 * i6 + 3.0 * c6 * i6) * inv;
-                    for (int k = 0; k < 3; ++k) {
-                      gc[k] += g * dv[k];
-                      gu[k] += g * o[si] * dv[k];
-                    }
-                  }
-                }
+      if (fp64) {
+        for (int k = 0; k < 3; ++k) r[k] = IG.list64[3 * i + k] - c[k];
+        const double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        if (!(rr < keep2)) continue;
+        for (int si = 0; si < 2; ++si) {
+          const double dv[3] = {o[si] * u[0] - r[0], o[si] * u[1] - r[1], o[si] * u[2] - r[2]};
+          const double d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+          if (d2 < rc2) {
+            const bool floored = d2 < fl2;
+            const double inv = 1.0 / (floored ? fl2 : d2);
+            const double i6 = inv * inv * inv;
+            E += c12 * i6 * i6 - c6 * i6 + sh;
+            if (GRAD && !floored) {
+              const double g = 2.0 * (-6.0 * c12 * i6 * i6 + 3.0 * c6 * i6) * inv;
+              for (int k = 0; k < 3; ++k) {
+                gc[k] += g * dv[k];
+                gu[k] += g * o[si] * dv[k];
               }
-            } else {
-              // FP32 pair arithmetic, clamped to [floor2, rc^2] (a clamped
-              // pair gives -shift, compensated by the +shift added per pair)
-              const float rf[3] = {(float)r[0], (float)r[1], (float)r[2]};
-              const float rrf = rf[0] * rf[0] + rf[1] * rf[1] + rf[2] * rf[2];
-              const float tt = rf[0] * uf[0] + rf[1] * uf[1] + rf[2] * uf[2];
-              float e = 0.f;
-              for (int si = 0; si < 2; ++si) {
-                const float of = (float)o[si];
-                const float d2r = fmaf(tt, -2.0f * of, rrf + of * of);
-                const float d2 = __builtin_amdgcn_fmed3f(d2r, fl2f, J.rc2f);
-                const float i = __builtin_amdgcn_rcpf(d2);
-                const float i3 = i * i * i;
-                e = fmaf(i3, fmaf(i3, s6, -1.0f), e);
-                if (GRAD && d2r > fl2f && d2r < J.rc2f) {
-                  const double g = 6.0 * c6 * (double)(i3 * i * fmaf(-2.0f * s6, i3, 1.0f));
-                  for (int k = 0; k < 3; ++k) {
-                    const double dk = o[si] * u[k] - r[k];
-                    gc[k] += g * dk;
-                    gu[k] += g * o[si] * dk;
-                  }
-                }
-              }
-              E += fma(c6, (double)e, 2.0 * sh);
             }
           }
+        }
+      } else {
+        // FP32 pair arithmetic, clamped to [floor2, rc^2] (a clamped pair gives
+        // -shift, compensated by the +shift added per candidate)
+        const float tt = rf[0] * uf[0] + rf[1] * uf[1] + rf[2] * uf[2];
+        float e = 0.f;
+        for (int si = 0; si < 2; ++si) {
+          const float of = (float)o[si];
+          const float d2r = fmaf(tt, -2.0f * of, rrf + of * of);
+          const float d2 = __builtin_amdgcn_fmed3f(d2r, fl2f, J.rc2f);
+          const float iv = __builtin_amdgcn_rcpf(d2);
+          const float i3 = iv * iv * iv;
+          e = fmaf(i3, fmaf(i3, s6, -1.0f), e);
+          if (GRAD && d2r > fl2f && d2r < J.rc2f) {
+            const double g = 6.0 * c6 * (double)(i3 * iv * fmaf(-2.0f * s6, i3, 1.0f));
+            for (int k = 0; k < 3; ++k) {
+              const double dk = o[si] * u[k] - (double)rf[k];
+              gc[k] += g * dk;
+              gu[k] += g * o[si] * dk;
+            }
+          }
+        }
+        E += fma(c6, (double)e, 2.0 * sh);
+      }
     }
   }
   E = wave_sum_d(E);
@@ -173,7 +207,7 @@ __global__ __launch_bounds__(256) void image_kernel(const Job* __restrict__ plan
 }
 
 // Steps 1-4: forces -> s1 -> endpoint fix -> reparametrized s2.
-__global__ __launch_bounds__(256) void string_step_kernel(const Job* __restrict__ plans,
+__global__ __launch_bounds__(256) void string_step_kernel(const ImageGrid* __restrict__ grids,
                                                           const int* __restrict__ plan_of,
                                                           const int* __restrict__ direction,
                                                           StringParams sp,
@@ -187,7 +221,7 @@ __global__ __launch_bounds__(256) void string_step_kernel(const Job* __restrict_
   const int N = sp.n_images;
   double* s1 = sh;            // [N][6]
   double* cum = sh + 6 * N;   // [N]
-  const Job& J = plans[plan_of[s]];
+  const Job& J = *grids[plan_of[s]].job;
   const int base = s * N;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const int p = base + i;
@@ -203,12 +237,20 @@ __global__ __launch_bounds__(256) void string_step_kernel(const Job* __restrict_
     }
   }
   __syncthreads();
+  const int d = direction[s] - 1;
   if (threadIdx.x == 0) {
-    const int d = direction[s] - 1;
     s1[d] = x0[6 * base + d];
     s1[6 * (N - 1) + d] = x0[6 * (base + N - 1) + d];
     for (int k = 0; k < 6; ++k)
       if (k != d) s1[6 * (N - 1) + k] = s1[k];
+  }
+  __syncthreads();
+  // Interior images outside [0, 1] along the hop direction are rare; the
+  // in-order replacement runs only when some image needs it.
+  int outside = 0;
+  for (int i = 1 + threadIdx.x; i < N - 1; i += blockDim.x)
+    outside |= (s1[6 * i + d] < 0.0) | (s1[6 * i + d] > 1.0);
+  if (__syncthreads_or(outside) && threadIdx.x == 0) {
     for (int i = 1; i < N - 1; ++i) {
       int from = -1;
       if (s1[6 * i + d] < 0.0) {
@@ -221,20 +263,24 @@ __global__ __launch_bounds__(256) void string_step_kernel(const Job* __restrict_
       if (from >= 0)
         for (int k = 0; k < 6; ++k) s1[6 * i + k] = s1[6 * from + k];
     }
-    // cumulative Cartesian arc length of (a, b, c)
-    cum[0] = 0.0;
-    double prev[3];
-    for (int k = 0; k < 3; ++k)
-      prev[k] = s1[0] * J.H[k] + s1[1] * J.H[3 + k] + s1[2] * J.H[6 + k];
-    for (int i = 1; i < N; ++i) {
-      double cur[3], l2 = 0.0;
-      for (int k = 0; k < 3; ++k) {
-        cur[k] = s1[6 * i] * J.H[k] + s1[6 * i + 1] * J.H[3 + k] + s1[6 * i + 2] * J.H[6 + k];
-        l2 += (cur[k] - prev[k]) * (cur[k] - prev[k]);
-        prev[k] = cur[k];
-      }
-      cum[i] = cum[i - 1] + sqrt(l2);
+  }
+  __syncthreads();
+  // Cartesian segment lengths of (a, b, c) in parallel, then the cumulative
+  // sum in image order.
+  for (int i = 1 + threadIdx.x; i < N; i += blockDim.x) {
+    double l2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double prev = s1[6 * (i - 1)] * J.H[k] + s1[6 * (i - 1) + 1] * J.H[3 + k] +
+                          s1[6 * (i - 1) + 2] * J.H[6 + k];
+      const double cur = s1[6 * i] * J.H[k] + s1[6 * i + 1] * J.H[3 + k] + s1[6 * i + 2] * J.H[6 + k];
+      l2 += (cur - prev) * (cur - prev);
     }
+    cum[i] = sqrt(l2);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cum[0] = 0.0;
+    for (int i = 1; i < N; ++i) cum[i] = cum[i - 1] + cum[i];
   }
   __syncthreads();
   const double total = cum[N - 1];
@@ -337,7 +383,7 @@ __global__ __launch_bounds__(256) void string_finish_kernel(StringParams sp,
   if (threadIdx.x == 0) iterations[s] = it + 1;
 }
 
-hipError_t launch_image(int precision, bool grad, const Job* plans, const int* plan_of, int N,
+hipError_t launch_image(int precision, bool grad, const ImageGrid* plans, const int* plan_of, int N,
                         int P, const int* converged, const double* x, double* V, double* G,
                         hipStream_t st) {
   const dim3 g((P + 3) / 4), b(256);
@@ -358,8 +404,8 @@ struct StringSys {
   hipStream_t stream = nullptr;
   int S = 0, N = 0, P = 0, precision = 0;
   StringParams sp{};
-  std::vector<Job> jobs;
-  DevBuf<Job> d_jobs;
+  std::vector<PoseGrid*> grids;      // candidate lists per distinct framework
+  DevBuf<ImageGrid> d_grids;
   DevBuf<int> d_plan_of, d_dir, d_conv, d_iter;
   DevBuf<double> d_x0, d_x2, d_G, d_V0, d_V2;
   hipGraph_t graph = nullptr;
@@ -368,6 +414,7 @@ struct StringSys {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);
+    for (PoseGrid* g : grids) delete g;
   }
 };
 
@@ -383,13 +430,13 @@ int iteration(StringSys* Y) {
   const hipStream_t st = Y->stream;
   const size_t step_shm = sizeof(double) * 7 * Y->N;
   const size_t fin_shm = sizeof(double) * (3 * Y->N + 2 * 256);
-  SV_HIP_S(launch_image(Y->precision, true, Y->d_jobs.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
+  SV_HIP_S(launch_image(Y->precision, true, Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
                         Y->d_x0.p, Y->d_V0.p, Y->d_G.p, st));
-  hipLaunchKernelGGL(string_step_kernel, dim3(Y->S), dim3(256), step_shm, st, Y->d_jobs.p,
+  hipLaunchKernelGGL(string_step_kernel, dim3(Y->S), dim3(256), step_shm, st, Y->d_grids.p,
                      Y->d_plan_of.p, Y->d_dir.p, Y->sp, Y->d_conv.p, Y->d_x0.p, Y->d_G.p,
                      Y->d_x2.p);
   SV_HIP_S(hipGetLastError());
-  SV_HIP_S(launch_image(Y->precision, false, Y->d_jobs.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
+  SV_HIP_S(launch_image(Y->precision, false, Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
                         Y->d_x2.p, Y->d_V2.p, nullptr, st));
   hipLaunchKernelGGL(string_finish_kernel, dim3(Y->S), dim3(256), fin_shm, st, Y->sp, Y->d_conv.p,
                      Y->d_iter.p, Y->d_x0.p, Y->d_x2.p, Y->d_V0.p, Y->d_V2.p);
@@ -429,22 +476,45 @@ int sv_strings_create(int n_strings, void* const* plans, int n_images, const dou
   Y->sp.conv_o = 0.30 * move_angle_rad * n_images * std::sqrt(3.0);
   std::map<const void*, int> index;
   std::vector<int> plan_of(n_strings);
+  Y->device = static_cast<Plan*>(plans[0])->device;
   for (int s = 0; s < n_strings; ++s) {
     auto it = index.find(plans[s]);
     if (it == index.end()) {
-      Job J;
-      std::memset(&J, 0, sizeof(Job));
-      plan_fill_job(*static_cast<Plan*>(plans[s]), J);
-      it = index.emplace(plans[s], (int)Y->jobs.size()).first;
-      Y->jobs.push_back(J);
+      Plan* P = static_cast<Plan*>(plans[s]);
+      if (P->device != Y->device) {
+        delete Y;
+        return fail(1, "all plans must be on one device");
+      }
+      // bins of about kBinEdge along every lattice direction
+      int m[3];
+      for (int k = 0; k < 3; ++k)
+        m[k] = (int)std::min(256.0, std::max(1.0, std::floor(1.0 / (P->hn[k] * kBinEdge) + 0.5)));
+      PoseGrid* g = nullptr;
+      const int rc = pose_grid_build(P, m, &g);
+      if (rc) {
+        delete Y;
+        return rc;
+      }
+      it = index.emplace(plans[s], (int)Y->grids.size()).first;
+      Y->grids.push_back(g);
     }
     plan_of[s] = it->second;
   }
-  Y->device = static_cast<Plan*>(plans[0])->device;
   std::vector<int> zeros(n_strings, 0);
   hipError_t e;
+  std::vector<ImageGrid> ig(Y->grids.size());
+  for (size_t i = 0; i < Y->grids.size(); ++i) {
+    const PoseGrid* g = Y->grids[i];
+    ig[i] = ImageGrid{};
+    ig[i].job = g->B->d_jobs.p;
+    std::memcpy(ig[i].Hinv, g->Hinv.m, sizeof(ig[i].Hinv));
+    for (int k = 0; k < 3; ++k) ig[i].m[k] = g->m[k];
+    ig[i].offsets = g->B->d_offsets.p;
+    ig[i].list32 = g->B->d_list32.p;
+    ig[i].list64 = g->B->d_list64.p;
+  }
   if ((e = hipSetDevice(Y->device)) != hipSuccess || (e = hipStreamCreate(&Y->stream)) != hipSuccess ||
-      (e = Y->d_jobs.upload(Y->jobs.data(), Y->jobs.size(), Y->stream)) != hipSuccess ||
+      (e = Y->d_grids.upload(ig.data(), ig.size(), Y->stream)) != hipSuccess ||
       (e = Y->d_plan_of.upload(plan_of.data(), n_strings, Y->stream)) != hipSuccess ||
       (e = Y->d_dir.upload(direction, n_strings, Y->stream)) != hipSuccess ||
       (e = Y->d_conv.upload(zeros.data(), n_strings, Y->stream)) != hipSuccess ||
@@ -495,7 +565,7 @@ int sv_strings_state(void* strings, double* path, double* energy, int* iteration
   StringSys* Y = static_cast<StringSys*>(strings);
   SV_HIP_S(hipSetDevice(Y->device));
   if (energy)
-    SV_HIP_S(launch_image(Y->precision, false, Y->d_jobs.p, Y->d_plan_of.p, Y->N, Y->P, nullptr,
+    SV_HIP_S(launch_image(Y->precision, false, Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, nullptr,
                           Y->d_x0.p, Y->d_V0.p, nullptr, Y->stream));
   if (path) SV_HIP_S(hipMemcpyAsync(path, Y->d_x0.p, sizeof(double) * 6 * Y->P, hipMemcpyDeviceToHost, Y->stream));
   if (energy) SV_HIP_S(hipMemcpyAsync(energy, Y->d_V0.p, sizeof(double) * Y->P, hipMemcpyDeviceToHost, Y->stream));
