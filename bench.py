@@ -339,11 +339,14 @@ def workload_strings(args):
             "converged_mixed": int(cmx.sum()),
             "barriers_fp64_K": [round(float(v), 6) for v in b64],
             "barriers_mixed_K": [round(float(v), 6) for v in bmx]}
+    # The FP64 form is the faster one here (FP64 issue runs at the FP32 rate and
+    # the image sweep is short), so it is the reported value; the mixed form's
+    # time and its same-path error are reported beside it.
     line = {"metric": "string iterations/s, 64 strings x 96 images, hipGraph",
-            "value": 1e3 / report["full"]["ms_per_iteration_mixed"], "unit": "iterations/s",
+            "value": 1e3 / report["full"]["ms_per_iteration_fp64"], "unit": "iterations/s",
             "n_gpus": 1, "steps": iters, "warmup": 1,
-            "ms_per_step": report["full"]["ms_per_iteration_mixed"], "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "ms_per_step": report["full"]["ms_per_iteration_fp64"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "64 strings x 96 images (8 frameworks x 8 strings, straight "
                                    "fractional paths along a/b/c), 2000 iterations, one iteration "
                                    "per HIP graph replay"},

@@ -776,11 +776,9 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
 // order of poses inside a bin depends on atomic timing, which changes no
 // result: each pose is evaluated by its own lane over its bin's list.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wrap01_dev(double x) {
-  double r = fmod(x, 1.0);
-  if (r != 0.0 && r < 0.0) r += 1.0;
-  return r;
-}
+// x - floor(x): the same value as fmod(x, 1) (+1 when negative) -- both
+// subtractions are exact or round alike -- at one v_floor_f64.
+__device__ __forceinline__ double wrap01_dev(double x) { return x - floor(x); }
 
 __global__ __launch_bounds__(256) void pose_bin_kernel(int n, const double* __restrict__ centers,
                                                        const double* __restrict__ axes, Mat3 Hinv,

@@ -59,6 +59,16 @@ struct StringParams {
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+__device__ __forceinline__ void wave_barrier_s() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -80,11 +90,9 @@ struct ImageGrid {
   const double* list64;
 };
 
-__device__ __forceinline__ double wrap01_s(double x) {
-  double r = fmod(x, 1.0);
-  if (r != 0.0 && r < 0.0) r += 1.0;
-  return r;
-}
+// x - floor(x): the same value as fmod(x, 1) (+1 when negative) -- both
+// subtractions are exact or round alike -- at one v_floor_f64.
+__device__ __forceinline__ double wrap01_s(double x) { return x - floor(x); }
 
 // Energy (and gradient) of one pose per wave.  x = (a, b, c, alpha, beta,
 // gamma) with fractional center (a, b, c).  The image is located in its bin;
@@ -95,15 +103,20 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
                                                     int n_poses, const int* __restrict__ converged,
                                                     const double* __restrict__ x,
                                                     double* __restrict__ V, double* __restrict__ G) {
+  __shared__ int s_near[4][128];   // queued near-contact candidates per wave
   const int lane = threadIdx.x & 63;
-  const int p = blockIdx.x * 4 + uniform_i(threadIdx.x >> 6);
+  const int wv = uniform_i(threadIdx.x >> 6);
+  const int p = blockIdx.x * 4 + wv;
   if (p >= n_poses) return;
   const int s = p / n_images;
   if (converged && converged[s]) return;
   const ImageGrid& IG = grids[plan_of[s]];
   const Job& J = *IG.job;
   const double be = x[6 * p + 4], ga = x[6 * p + 5];
-  const double u[3] = {cos(ga) * cos(be), sin(ga) * cos(be), -sin(be)};
+  double sb, cb, sg, cg;
+  sincos(be, &sb, &cb);
+  sincos(ga, &sg, &cg);
+  const double u[3] = {cg * cb, sg * cb, -sb};
   const float uf[3] = {(float)u[0], (float)u[1], (float)u[2]};
   // bin of the center and its offset from the bin centre (fractional differences)
   int idx[3];
@@ -124,68 +137,120 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
   const double o[2] = {J.o0, J.o1};
   const double keep2 = J.keep_r * J.keep_r, rc2 = J.rc2;
   double E = 0.0, gc[3] = {0.0, 0.0, 0.0}, gu[3] = {0.0, 0.0, 0.0};
+  // FP64 pair terms of candidate i (reference semantics, both sites).
+  auto pair64r = [&](const double* q, double c12, double c6, double sh, double fl2) {
+    double r[3];
+    for (int k = 0; k < 3; ++k) r[k] = q[k] - c[k];
+    const double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    if (!(rr < keep2)) return;
+    for (int si = 0; si < 2; ++si) {
+      const double dv[3] = {o[si] * u[0] - r[0], o[si] * u[1] - r[1], o[si] * u[2] - r[2]};
+      const double d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+      if (d2 < rc2) {
+        const bool floored = d2 < fl2;
+        const double inv = 1.0 / (floored ? fl2 : d2);
+        const double i6 = inv * inv * inv;
+        E += c12 * i6 * i6 - c6 * i6 + sh;
+        if (GRAD && !floored) {
+          const double g = 2.0 * (-6.0 * c12 * i6 * i6 + 3.0 * c6 * i6) * inv;
+          for (int k = 0; k < 3; ++k) {
+            gc[k] += g * dv[k];
+            gu[k] += g * o[si] * dv[k];
+          }
+        }
+      }
+    }
+  };
+  auto pair64 = [&](long long i, double c12, double c6, double sh, double fl2) {
+    const double q[3] = {IG.list64[3 * i], IG.list64[3 * i + 1], IG.list64[3 * i + 2]};
+    pair64r(q, c12, c6, sh, fl2);
+  };
   for (int t = 0; t < J.n_types; ++t) {
     const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
     const long long beg = IG.offsets[seg0 + t], end = IG.offsets[seg0 + t + 1];
-    for (long long i = beg + lane; i < end; i += 64) {
-      bool fp64 = !MIXED;
-      double r[3];
-      float rf[3], rrf = 0.0f;
-      if (MIXED) {
-        const float4 e = IG.list32[i];
+    if (!MIXED) {
+      // the next candidate's coordinates are loaded while this one is evaluated
+      long long i = beg + lane;
+      double q[3] = {0.0, 0.0, 0.0};
+      if (i < end)
+        for (int k = 0; k < 3; ++k) q[k] = IG.list64[3 * i + k];
+      for (; i < end; i += 64) {
+        const double cur[3] = {q[0], q[1], q[2]};
+        if (i + 64 < end)
+          for (int k = 0; k < 3; ++k) q[k] = IG.list64[3 * (i + 64) + k];
+        pair64r(cur, c12, c6, sh, fl2);
+      }
+      continue;
+    }
+    // Mixed: FP32 terms in the main sweep; near-contact candidates are queued
+    // in LDS and evaluated in FP64 one per lane, so the sweep never branches
+    // into the FP64 path.
+    int nq = 0;
+    // Lane partials of this type in FP32: e in units of c6, gradient in K/A;
+    // flushed into the FP64 sums once per type.
+    float eacc = 0.f, gcf[3] = {0.f, 0.f, 0.f}, guf[3] = {0.f, 0.f, 0.f};
+    int kept = 0;
+    const float c6f6 = (float)(6.0 * c6);
+    const float of0 = (float)o[0], of1 = (float)o[1];
+    // the next chunk's entry is loaded while this chunk is evaluated
+    float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (beg + lane < end) next = IG.list32[beg + lane];
+    for (long long b0 = beg; b0 < end; b0 += 64) {
+      const long long i = b0 + lane;
+      const float4 e = next;
+      if (i + 64 < end) next = IG.list32[i + 64];
+      float rf[3] = {0.f, 0.f, 0.f}, rrf = 3.0e38f;
+      if (i < end) {
         rf[0] = e.x - cf[0];
         rf[1] = e.y - cf[1];
         rf[2] = e.z - cf[2];
         rrf = rf[0] * rf[0] + rf[1] * rf[1] + rf[2] * rf[2];
-        if (!(rrf < J.keep2f)) continue;
-        fp64 = rrf < near2;
       }
-      if (fp64) {
-        for (int k = 0; k < 3; ++k) r[k] = IG.list64[3 * i + k] - c[k];
-        const double rr = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-        if (!(rr < keep2)) continue;
-        for (int si = 0; si < 2; ++si) {
-          const double dv[3] = {o[si] * u[0] - r[0], o[si] * u[1] - r[1], o[si] * u[2] - r[2]};
-          const double d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
-          if (d2 < rc2) {
-            const bool floored = d2 < fl2;
-            const double inv = 1.0 / (floored ? fl2 : d2);
-            const double i6 = inv * inv * inv;
-            E += c12 * i6 * i6 - c6 * i6 + sh;
-            if (GRAD && !floored) {
-              const double g = 2.0 * (-6.0 * c12 * i6 * i6 + 3.0 * c6 * i6) * inv;
-              for (int k = 0; k < 3; ++k) {
-                gc[k] += g * dv[k];
-                gu[k] += g * o[si] * dv[k];
-              }
-            }
-          }
-        }
-      } else {
-        // FP32 pair arithmetic, clamped to [floor2, rc^2] (a clamped pair gives
-        // -shift, compensated by the +shift added per candidate)
+      const bool keep = rrf < J.keep2f;
+      const bool near = keep && rrf < near2;
+      const unsigned long long mnear = __ballot(near);
+      if (near) s_near[wv][nq + lanes_below(mnear)] = (int)(i - beg);
+      nq += __popcll(mnear);
+      if (keep && !near) {
+        // clamped to [floor2, rc^2]: a clamped pair gives -shift, compensated by
+        // the +shift added per candidate
+        ++kept;
         const float tt = rf[0] * uf[0] + rf[1] * uf[1] + rf[2] * uf[2];
-        float e = 0.f;
+#pragma unroll
         for (int si = 0; si < 2; ++si) {
-          const float of = (float)o[si];
+          const float of = si ? of1 : of0;
           const float d2r = fmaf(tt, -2.0f * of, rrf + of * of);
           const float d2 = __builtin_amdgcn_fmed3f(d2r, fl2f, J.rc2f);
           const float iv = __builtin_amdgcn_rcpf(d2);
           const float i3 = iv * iv * iv;
-          e = fmaf(i3, fmaf(i3, s6, -1.0f), e);
-          if (GRAD && d2r > fl2f && d2r < J.rc2f) {
-            const double g = 6.0 * c6 * (double)(i3 * iv * fmaf(-2.0f * s6, i3, 1.0f));
+          eacc = fmaf(i3, fmaf(i3, s6, -1.0f), eacc);
+          if (GRAD) {
+            const bool in = d2r > fl2f && d2r < J.rc2f;
+            const float g = in ? c6f6 * (i3 * iv * fmaf(-2.0f * s6, i3, 1.0f)) : 0.0f;
+#pragma unroll
             for (int k = 0; k < 3; ++k) {
-              const double dk = o[si] * u[k] - (double)rf[k];
-              gc[k] += g * dk;
-              gu[k] += g * o[si] * dk;
+              const float dk = fmaf(of, uf[k], -rf[k]);
+              gcf[k] = fmaf(g, dk, gcf[k]);
+              guf[k] = fmaf(g * of, dk, guf[k]);
             }
           }
         }
-        E += fma(c6, (double)e, 2.0 * sh);
+      }
+      if (nq >= 64 || b0 + 64 >= end) {
+        wave_barrier_s();
+        for (int q0 = 0; q0 < nq; q0 += 64)
+          if (q0 + lane < nq) pair64(beg + s_near[wv][q0 + lane], c12, c6, sh, fl2);
+        wave_barrier_s();
+        nq = 0;
       }
     }
+    E += fma(c6, (double)eacc, 2.0 * sh * kept);
+    if (GRAD)
+      for (int k = 0; k < 3; ++k) {
+        gc[k] += (double)gcf[k];
+        gu[k] += (double)guf[k];
+      }
   }
   E = wave_sum_d(E);
   if (GRAD)
@@ -200,8 +265,8 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
       for (int k = 0; k < 3; ++k)
         G[6 * p + k] = J.H[3 * k] * gc[0] + J.H[3 * k + 1] * gc[1] + J.H[3 * k + 2] * gc[2];
       G[6 * p + 3] = 0.0;
-      G[6 * p + 4] = gu[0] * (-cos(ga) * sin(be)) + gu[1] * (-sin(ga) * sin(be)) + gu[2] * (-cos(be));
-      G[6 * p + 5] = gu[0] * (-sin(ga) * cos(be)) + gu[1] * (cos(ga) * cos(be));
+      G[6 * p + 4] = gu[0] * (-cg * sb) + gu[1] * (-sg * sb) + gu[2] * (-cb);
+      G[6 * p + 5] = gu[0] * (-sg * cb) + gu[1] * (cg * cb);
     }
   }
 }
