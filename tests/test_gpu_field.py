@@ -152,3 +152,21 @@ def test_arbitrary_centers_and_many_axes(oracle, small_case):
     assert rel.max() < 1e-12
     np.testing.assert_allclose(out["marginal"], oracle.marginal_energy(ref, 298.0),
                                rtol=1e-6, atol=1e-3)
+
+
+@pytest.mark.parametrize("near_energy", [0.0, 5.0, 1000.0])
+def test_mixed_near_contact_bound_settings(oracle, small_case, near_energy):
+    """near_energy_K <= 0 turns the FP64 near-contact path off (every kept pair
+    in FP32, the floor handled by the clamp); other values move the energy
+    bound (the wall-slope bound still applies).  With the path off the
+    accessible error exceeds 1e-3 K (1.1e-3 K here): that is what the FP64
+    near-contact terms are for."""
+    from string_vext_amd.field import Plan
+    size, k = 6, 64
+    centers, axes = oracle.regular_pose_grid(size, k)
+    ref = oracle.reuse_energy(small_case, centers, axes)
+    plan = Plan.from_case(small_case, near_energy_K=near_energy)
+    out = plan.field(plan.grid(size, axes), precision="mixed")["energy"]
+    m = oracle.error_metrics(ref, out)
+    assert m["accessible_max_abs_error_K"] < (1e-3 if near_energy > 0 else 1e-2)
+    assert m["all_max_relative_error_floor1K"] < 1e-4
