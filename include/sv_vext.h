@@ -177,6 +177,18 @@ int sv_pose_grid_eval(void *grid, int n, const double *centers_cart, const doubl
                       double *grad_center, double *grad_axis);
 int sv_pose_grid_info(void *grid, int *n_tiles, int *capacity);
 int sv_pose_grid_destroy(void *grid);
+/* Asynchronous form: enqueues the same evaluation on `stream` (a hipStream_t; */
+/* NULL = the default stream) and returns without waiting, so the outputs    */
+/* are ordered with the caller's other work on that stream.  Calls on one    */
+/* grid are serialized: a call on a different stream first waits for the    */
+/* previous one.  Invalid input and nonfinite results are not reported by   */
+/* the call; they accumulate in the grid's flags, which sv_pose_grid_check   */
+/* returns (1 or 4, as above; after waiting for the grid's last stream) and  */
+/* clears.  Results of invalid poses are NaN.                                 */
+int sv_pose_grid_eval_async(void *grid, void *stream, int n, const double *centers_cart,
+                            const double *axes, int precision, const double *grad_out,
+                            double *energy, double *grad_center, double *grad_axis);
+int sv_pose_grid_check(void *grid);
 
 /* ------------------------------------------------------------------------ */
 /* Batched String method: n_strings independent strings of n_images images. */

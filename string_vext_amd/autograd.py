@@ -6,8 +6,11 @@
 ``centers`` are Cartesian (Angstrom), ``axes`` the directors, both (n, 3)
 CUDA tensors; directors are used as given (normalize them in torch if they are
 free parameters, and autograd carries the normalization).  The poses stay on
-the device: each call bins them into the plan's cached tile grid on the GPU
-(sv_pose_grid_eval).  Forward runs the energy kernel; backward runs a separate
+the device: each call bins them into the plan's cached tile grid on the GPU,
+enqueued on torch's current stream without a host synchronization
+(sv_pose_grid_eval_async); invalid poses give NaN energies, like other torch
+operations, and ``check_poses(plan)`` raises on them.  Forward runs the energy
+kernel; backward runs a separate
 kernel that recomputes every pair term and returns dE/dc and dE/du scaled by
 the upstream gradient.  There is no counterpart in the reference, whose String
 forces are forward differences of the energy
@@ -41,8 +44,9 @@ class PoseEnergy(torch.autograd.Function):
         u = axes.detach().to(device=device, dtype=torch.float64).contiguous()
         energy = torch.empty(len(c), dtype=torch.float64, device=device)
         grid = _grid_for(plan, len(c))
-        torch.cuda.current_stream(device).synchronize()
-        grid.eval(len(c), c.data_ptr(), u.data_ptr(), precision, energy=energy.data_ptr())
+        stream = torch.cuda.current_stream(device).cuda_stream
+        grid.eval_async(stream, len(c), c.data_ptr(), u.data_ptr(), precision,
+                        energy=energy.data_ptr())
         ctx.save_for_backward(c, u)
         ctx.grid = grid
         ctx.precision = precision
@@ -56,10 +60,18 @@ class PoseEnergy(torch.autograd.Function):
         g = grad_energy.detach().to(device=c.device, dtype=torch.float64).contiguous()
         grad_c = torch.empty((n, 3), dtype=torch.float64, device=c.device)
         grad_u = torch.empty((n, 3), dtype=torch.float64, device=c.device)
-        torch.cuda.current_stream(c.device).synchronize()
-        ctx.grid.eval(n, c.data_ptr(), u.data_ptr(), ctx.precision, grad_out=g.data_ptr(),
-                      grad_center=grad_c.data_ptr(), grad_axis=grad_u.data_ptr())
+        stream = torch.cuda.current_stream(c.device).cuda_stream
+        ctx.grid.eval_async(stream, n, c.data_ptr(), u.data_ptr(), ctx.precision,
+                            grad_out=g.data_ptr(), grad_center=grad_c.data_ptr(),
+                            grad_axis=grad_u.data_ptr())
         return grad_c.to(ctx.dtypes[0]), grad_u.to(ctx.dtypes[1]), None, None
+
+
+def check_poses(plan: Plan) -> None:
+    """Raise if any pose evaluation since the last check had a nonfinite center,
+    a nonfinite or zero director, or a nonfinite result (waits for the GPU)."""
+    for grid in plan.__dict__.get("_pose_grids", {}).values():
+        grid.check()
 
 
 def pose_energy(plan: Plan, centers: torch.Tensor, axes: torch.Tensor,

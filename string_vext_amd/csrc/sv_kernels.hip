@@ -752,6 +752,15 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
   }
   if (!ok) return;
   const int cid = J.center_id[slot];
+  // A pose with a nonfinite center or a nonfinite / zero director (only device
+  // binned poses can carry one; it is flagged there) gets NaN results rather
+  // than the sum over the no pairs its NaN distances select.
+  const double unn = ux * ux + uy * uy + uz * uz;
+  const bool valid = isfinite(cx) && isfinite(cy) && isfinite(cz) && isfinite(unn) && unn > 0.0;
+  if (!valid) {
+    a.E = __builtin_nan("");
+    a.G0 = a.G1 = a.E;
+  }
   if (!isfinite(a.E)) atomicOr(J.bad, 1);
   if (J.pose_e) J.pose_e[cid] = a.E;
   if (GRAD) {
@@ -1015,6 +1024,24 @@ __global__ __launch_bounds__(256) void legacy_kernel(int n_centers, int n_axes, 
   }
 }
 
+__global__ void job_patch_kernel(Job* __restrict__ J, const double* center_rel,
+                                 const int* center_id, const int* tile_of, const double* axes64,
+                                 const float* axes32, int n, const double* grad_out,
+                                 double* pose_e, double* grad_c, double* grad_u) {
+  if (threadIdx.x == 0) {
+    J->center_rel = center_rel;
+    J->center_id = center_id;
+    J->tile_of = tile_of;
+    J->pose_axes64 = axes64;
+    J->pose_axes32 = axes32;
+    J->n_centers = n;
+    J->grad_out = grad_out;
+    J->pose_e = pose_e;
+    J->grad_c = grad_c;
+    J->grad_u = grad_u;
+  }
+}
+
 __global__ void zero_kernel(int* __restrict__ p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -1035,6 +1062,15 @@ __global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int n) {
 hipError_t launch_zero(int* p, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_job_patch(Job* d_job, const double* center_rel, const int* center_id,
+                            const int* tile_of, const double* axes64, const float* axes32, int n,
+                            const double* grad_out, double* pose_e, double* grad_c,
+                            double* grad_u, hipStream_t s) {
+  hipLaunchKernelGGL(job_patch_kernel, dim3(1), dim3(64), 0, s, d_job, center_rel, center_id,
+                     tile_of, axes64, axes32, n, grad_out, pose_e, grad_c, grad_u);
   return hipGetLastError();
 }
 

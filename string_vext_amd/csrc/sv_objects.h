@@ -122,8 +122,11 @@ struct PoseGrid {
   int capacity = 0;          // poses the slot arrays hold
   double radius = 0.0;       // circumscribed radius of one bin
   Mat3 H{}, Hinv{};
+  hipStream_t last_stream = nullptr;   // stream of the last enqueued evaluation
+  bool last_stream_set = false;
   DevBuf<int> bin, rank, counts, first, n_chunks;
   ~PoseGrid() {
+    if (last_stream_set) (void)hipStreamSynchronize(last_stream);
     delete B;
     delete S;
   }

@@ -607,7 +607,7 @@ extern "C" {
 
 const char* sv_last_error(void) { return t_error.c_str(); }
 
-int sv_abi_version(void) { return 2; }
+int sv_abi_version(void) { return 3; }
 
 int sv_device_count(void) {
   int n = 0;
@@ -1000,46 +1000,50 @@ int sv_pose_grid_create(void* plan, int n_expected, int tile_centers, void** gri
   return 0;
 }
 
-int sv_pose_grid_eval(void* grid, int n, const double* centers_cart, const double* axes,
-                      int precision, const double* grad_out, double* energy, double* grad_center,
-                      double* grad_axis) {
-  if (!grid) return fail(1, "null pose grid");
+}  // extern "C"
+
+namespace sv {
+namespace {
+
+// Enqueue one evaluation of a pose grid on stream st; no host synchronization.
+// Flags of invalid input (2) and nonfinite results (1) accumulate in the
+// batch's device flag.
+int pose_grid_enqueue(PoseGrid* G, hipStream_t st, int n, const double* centers_cart,
+                      const double* axes, int precision, const double* grad_out, double* energy,
+                      double* grad_center, double* grad_axis) {
   if (n < 1) return fail(1, "at least one pose required");
   if (!centers_cart || !axes) return fail(1, "null pose arrays");
   if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
-  PoseGrid* G = static_cast<PoseGrid*>(grid);
   Batch* B = G->B;
   Poses* S = G->S;
   SV_HIP(hipSetDevice(G->P->device));
   const long long max_chunks = (long long)G->n_bins + (n + 63) / 64;
   if (max_chunks > INT32_MAX) return fail(1, "too many poses");
-  if (n > G->capacity) {
-    const size_t c = (size_t)n + (size_t)n / 4;
-    SV_HIP(S->d_center_rel.reserve(3 * c));
-    SV_HIP(S->d_center_id.reserve(c));
-    SV_HIP(S->d_tile_of.reserve(c));
-    SV_HIP(S->d_pose_axes64.reserve(3 * c));
-    SV_HIP(S->d_pose_axes32.reserve(3 * c));
-    SV_HIP(G->bin.reserve(c));
-    SV_HIP(G->rank.reserve(c));
-    G->capacity = (int)std::min<size_t>(c, INT32_MAX);
+  // Work of earlier calls may still use the working buffers: it is ordered on
+  // its own stream, which is drained before a call on another stream or
+  // before the buffers are reallocated.
+  if (G->last_stream_set && G->last_stream != st) SV_HIP(hipStreamSynchronize(G->last_stream));
+  if (n > G->capacity || (size_t)max_chunks > B->d_chunks.cap) {
+    SV_HIP(hipStreamSynchronize(st));
+    if (n > G->capacity) {
+      const size_t c = (size_t)n + (size_t)n / 4;
+      SV_HIP(S->d_center_rel.reserve(3 * c));
+      SV_HIP(S->d_center_id.reserve(c));
+      SV_HIP(S->d_tile_of.reserve(c));
+      SV_HIP(S->d_pose_axes64.reserve(3 * c));
+      SV_HIP(S->d_pose_axes32.reserve(3 * c));
+      SV_HIP(G->bin.reserve(c));
+      SV_HIP(G->rank.reserve(c));
+      G->capacity = (int)std::min<size_t>(c, INT32_MAX);
+    }
+    SV_HIP(B->d_chunks.reserve((size_t)max_chunks, true));
   }
-  SV_HIP(B->d_chunks.reserve((size_t)max_chunks, true));
-  Job& J = B->jobs[0];
-  J.center_rel = S->d_center_rel.p;
-  J.center_id = S->d_center_id.p;
-  J.tile_of = S->d_tile_of.p;
-  J.pose_axes64 = S->d_pose_axes64.p;
-  J.pose_axes32 = S->d_pose_axes32.p;
-  J.n_centers = n;
-  J.grad_out = grad_out;
-  J.pose_e = energy;
-  J.grad_c = grad_center;
-  J.grad_u = grad_axis;
-  hipStream_t st = B->stream;
-  SV_HIP(hipMemcpyAsync(B->d_jobs.p, &J, sizeof(Job), hipMemcpyHostToDevice, st));
+  G->last_stream = st;
+  G->last_stream_set = true;
+  SV_HIP(launch_job_patch(B->d_jobs.p, S->d_center_rel.p, S->d_center_id.p, S->d_tile_of.p,
+                          S->d_pose_axes64.p, S->d_pose_axes32.p, n, grad_out, energy, grad_center,
+                          grad_axis, st));
   SV_HIP(launch_zero(G->counts.p, G->n_bins, st));
-  SV_HIP(launch_zero(B->d_bad.p, 1, st));
   SV_HIP(launch_pose_bin(n, centers_cart, axes, G->Hinv, G->m, G->bin.p, G->rank.p, G->counts.p,
                          B->d_bad.p, st));
   SV_HIP(launch_pose_tile_scan(G->counts.p, G->n_bins, 0, G->first.p, B->d_chunks.p,
@@ -1050,12 +1054,56 @@ int sv_pose_grid_eval(void* grid, int n, const double* centers_cart, const doubl
   const bool grad = grad_center || grad_axis;
   SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)max_chunks, G->n_chunks.p,
                      B->d_offsets.p, B->d_list32.p, B->d_list64.p, st));
+  return 0;
+}
+
+// Wait for the grid's last stream, report and clear its flags.
+int pose_grid_flags(PoseGrid* G) {
+  Batch* B = G->B;
+  const hipStream_t st = G->last_stream_set ? G->last_stream : B->stream;
   int bad = 0;
   SV_HIP(hipMemcpyAsync(&bad, B->d_bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  SV_HIP(launch_zero(B->d_bad.p, 1, st));
   SV_HIP(hipStreamSynchronize(st));
   if (bad & 2) return fail(1, "nonfinite pose center or director, or zero director");
   if (bad) return fail(4, "nonfinite pose energy or gradient");
   return 0;
+}
+
+}  // namespace
+}  // namespace sv
+
+extern "C" {
+
+int sv_pose_grid_eval(void* grid, int n, const double* centers_cart, const double* axes,
+                      int precision, const double* grad_out, double* energy, double* grad_center,
+                      double* grad_axis) {
+  if (!grid) return fail(1, "null pose grid");
+  PoseGrid* G = static_cast<PoseGrid*>(grid);
+  SV_HIP(hipSetDevice(G->P->device));
+  // flags of earlier asynchronous calls are not this call's
+  if (G->last_stream_set) SV_HIP(hipStreamSynchronize(G->last_stream));
+  SV_HIP(launch_zero(G->B->d_bad.p, 1, G->B->stream));
+  const int rc = pose_grid_enqueue(G, G->B->stream, n, centers_cart, axes, precision, grad_out,
+                                   energy, grad_center, grad_axis);
+  if (rc) return rc;
+  return pose_grid_flags(G);
+}
+
+int sv_pose_grid_eval_async(void* grid, void* stream, int n, const double* centers_cart,
+                            const double* axes, int precision, const double* grad_out,
+                            double* energy, double* grad_center, double* grad_axis) {
+  if (!grid) return fail(1, "null pose grid");
+  return pose_grid_enqueue(static_cast<PoseGrid*>(grid), static_cast<hipStream_t>(stream), n,
+                           centers_cart, axes, precision, grad_out, energy, grad_center,
+                           grad_axis);
+}
+
+int sv_pose_grid_check(void* grid) {
+  if (!grid) return fail(1, "null pose grid");
+  PoseGrid* G = static_cast<PoseGrid*>(grid);
+  SV_HIP(hipSetDevice(G->P->device));
+  return pose_grid_flags(G);
 }
 
 int sv_pose_grid_info(void* grid, int* n_tiles, int* capacity) {

@@ -303,6 +303,19 @@ class PoseGrid:
                                               precision_code(precision), grad_out, energy,
                                               grad_center, grad_axis))
 
+    def eval_async(self, stream, n, centers, axes, precision="fp64", grad_out=None, energy=None,
+                   grad_center=None, grad_axis=None):
+        """Enqueue on ``stream`` (a raw hipStream_t handle, e.g. torch's
+        ``current_stream().cuda_stream``; 0 = default stream) without waiting.
+        Invalid input and nonfinite results are reported by ``check``."""
+        _lib.check(self.lib.sv_pose_grid_eval_async(self.handle, stream or None, int(n), centers,
+                                                    axes, precision_code(precision), grad_out,
+                                                    energy, grad_center, grad_axis))
+
+    def check(self):
+        """Wait for the last enqueued evaluation; raise on flagged input/results."""
+        _lib.check(self.lib.sv_pose_grid_check(self.handle))
+
     def info(self):
         tiles, cap = ctypes.c_int(), ctypes.c_int()
         _lib.check(self.lib.sv_pose_grid_info(self.handle, ctypes.byref(tiles), ctypes.byref(cap)))

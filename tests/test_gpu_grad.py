@@ -231,3 +231,43 @@ def test_pose_grid_grows_shrinks_and_rejects_nonfinite_poses(oracle, case, poses
     c = torch.tensor(centers[:8], dtype=torch.float64, device="cuda")
     grid.eval(8, c.data_ptr(), u.data_ptr(), "fp64", energy=e.data_ptr())
     assert np.isfinite(e.cpu().numpy()).all()
+
+
+def test_pose_grid_async_on_torch_stream_and_flag_check(oracle, case, poses):
+    """sv_pose_grid_eval_async enqueues on the caller's stream with no host
+    wait; its results equal the synchronous call, and invalid input is
+    reported later by sv_pose_grid_check."""
+    import torch
+    from string_vext_amd.autograd import check_poses, pose_energy
+    from string_vext_amd.field import PoseGrid
+    centers, axes = poses
+    n = len(centers)
+    plan = _plan(case)
+    grid = PoseGrid(plan, n_expected=n)
+    c = torch.tensor(centers, dtype=torch.float64, device="cuda")
+    u = torch.tensor(axes, dtype=torch.float64, device="cuda")
+    e_sync = torch.empty(n, dtype=torch.float64, device="cuda")
+    grid.eval(n, c.data_ptr(), u.data_ptr(), "mixed", energy=e_sync.data_ptr())
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        e_async = torch.empty(n, dtype=torch.float64, device="cuda")
+        grid.eval_async(side.cuda_stream, n, c.data_ptr(), u.data_ptr(), "mixed",
+                        energy=e_async.data_ptr())
+        doubled = e_async * 2.0            # ordered after the evaluation on the same stream
+    side.synchronize()
+    grid.check()
+    assert torch.equal(e_async, e_sync)
+    assert torch.equal(doubled, 2.0 * e_sync)
+    bad = c.clone()
+    bad[5, 0] = float("inf")
+    e_bad = torch.empty(n, dtype=torch.float64, device="cuda")
+    grid.eval_async(torch.cuda.current_stream().cuda_stream, n, bad.data_ptr(), u.data_ptr(),
+                    "fp64", energy=e_bad.data_ptr())
+    with pytest.raises(RuntimeError, match="code 1"):
+        grid.check()
+    grid.check()                            # flags were cleared
+    # autograd path: no host wait; invalid poses show as NaN and in check_poses
+    e = pose_energy(plan, bad, u, "fp64")
+    assert torch.isnan(e[5]) or not torch.isfinite(e[5])
+    with pytest.raises(RuntimeError, match="code 1"):
+        check_poses(plan)
