@@ -34,7 +34,7 @@ struct Job {
   float keep2f;         // keep_r^2 in FP32
   float m2o0f, m2o1f;   // -2*o0, -2*o1
   float o0sqf;          // o0^2
-  float dR2f;           // o1^2 - o0^2 (0 for a symmetric guest)
+  float o1sqf;          // o1^2
   float in2f;           // (rc - max|o| - margin)^2: closer atoms are inside the
                         // cutoff for every orientation
   int sym;              // o1 == -o0 exactly

@@ -22,11 +22,16 @@
 // rc, a clamped out-of-range pair contributes exactly -shift, which the
 // per-type count of kept pairs adds back in FP64; no cutoff branch or select
 // is needed and a pair on the wrong side of rc by FP32 rounding changes the
-// energy by ~1e-8 K.  Per-chunk FP32 partials are flushed into an FP64
-// accumulator.  Atoms closer to the center than a per-type near radius (where
-// a single pair term can exceed the configured bound, default 50 K) are
-// evaluated in FP64 with the reference formula, so overlap poses keep FP64
-// relative accuracy.
+// energy by ~1e-8 K.  FP32 partials of at most kFlush atoms are flushed into
+// an FP64 accumulator.  Atoms closer to the center than a per-type near
+// radius -- where a single pair term can exceed the configured bound (default
+// 50 K) or the wall's slope exceeds 300 K/A, so that an FP32 distance error
+// would matter -- are evaluated in FP64 with the reference formula, staged in
+// LDS by the compacting lane; overlap poses keep FP64 relative accuracy.
+//
+// Matrix cores are not used: the distances are a 16x16x4 FP32 matrix product
+// per 16 atoms x 16 orientations, but FP32 MFMA does not co-execute with the
+// VALU on gfx950 and such a kernel measured slower than this one.
 #include "sv_internal.h"
 #include <climits>
 
@@ -399,7 +404,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
                cz = J.center_rel[3 * c.slot + 2];
   const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
-  const float o0sq = J.o0sqf, o1sq = (float)(J.o1 * J.o1), in2 = J.in2f;
+  const float o0sq = J.o0sqf, o1sq = J.o1sqf, in2 = J.in2f;
   const double rc2 = J.rc2;
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
   const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1;
@@ -662,7 +667,7 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
   const float uxf = (float)ux, uyf = (float)uy, uzf = (float)uz;
   const double o0 = J.o0, o1 = J.o1, rc2 = J.rc2;
   const float rc2f = J.rc2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
-  const float o0sq = J.o0sqf, o1sq = (float)(J.o1 * J.o1);
+  const float o0sq = J.o0sqf, o1sq = J.o1sqf;
   const float o0f = (float)o0, o1f = (float)o1;
   const long long seg0 = J.seg_base + (long long)ch.tile * J.n_types;
   PoseAcc a = {};

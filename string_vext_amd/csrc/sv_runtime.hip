@@ -81,7 +81,7 @@ void plan_fill_job(const Plan& P, Job& J) {
   J.m2o0f = (float)(-2.0 * P.o0);
   J.m2o1f = (float)(-2.0 * P.o1);
   J.o0sqf = (float)(P.o0 * P.o0);
-  J.dR2f = (float)(P.o1 * P.o1 - P.o0 * P.o0);
+  J.o1sqf = (float)(P.o1 * P.o1);
   const double in_r = P.cutoff - std::max(std::fabs(P.o0), std::fabs(P.o1)) - 1e-3;
   J.in2f = in_r > 0.0 ? (float)(in_r * in_r) : 0.0f;
   J.sym = (P.o1 == -P.o0);
