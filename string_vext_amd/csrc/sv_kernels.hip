@@ -351,15 +351,13 @@ __device__ __forceinline__ double pair64(double o, double ux, double uy, double 
   return e;
 }
 
-// 1/x in FP64 from v_rcp_f64 and two Newton steps (a few ulp; the mixed
-// kernel's near-contact terms only need FP64-level relative accuracy, not
-// IEEE rounding).
+// 1/x in FP64 from v_rcp_f64 and one Newton step.  The step squares the
+// seed's relative error (1 - x y1 = (1 - x y0)^2), so even a seed good to only
+// 2^-20 gives ~1e-12; the mixed kernel's near-contact terms need FP64-level
+// relative accuracy (they are summed with FP32-path terms), not IEEE rounding.
 __device__ __forceinline__ double inv_fast(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
 }
 
 // Same pair term from a precomputed squared distance (mixed near path).
@@ -416,6 +414,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t], lo2 = J.lo2f[t];
     const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
     long long kept_far = 0;
+    float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
     for (long long base = beg; base < end; base += 64) {
       const long long idx = base + lane;
       const bool valid = idx < end;
@@ -460,7 +459,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         const float is = __builtin_amdgcn_rcpf(ds);
         const float is6 = is * is * is;
         const float sf = is6 * fmaf(is6, s6, -1.0f);
-        S = fma(c6, (double)(is_far || is_in ? sf : 0.0f), S);
+        Sf += is_far || is_in ? sf : 0.0f;
         if (is_near) S += pair64_d2(RRn, rc2, c12, c6, sh, fl2);
       }
       wave_sync();
@@ -521,6 +520,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     // Every kept far pair was clamped to rc^2 at worst, contributing -shift.
     E = fma((double)(2 * kept_far), sh, E);
     S_shift = fma((double)kept_far, sh, S_shift);
+    S = fma(c6, (double)Sf, S);
   }
   write_pose(c, E, do_site, lane == 0 ? S + S_shift : S);
 }
