@@ -379,6 +379,7 @@ struct MixedLds {
   float in1[4][64];    // |r|^2 + o1^2 (asymmetric guests only)
   float far1[4][64];
   double4 near[4][64]; // FP64 (qx, qy, qz, |q|^2) of near-contact atoms
+  int near_idx[4][64]; // queued near-contact list entries (offset in the type segment)
 };
 
 // One pose center x 64 orientations of the mixed-precision field.
@@ -415,6 +416,33 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
     long long kept_far = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
+    // Near-contact atoms are queued by list offset and evaluated when the queue
+    // would overflow and at the end of the type: the compacting lanes stage
+    // their FP64 center-relative vectors (and site terms) lane-parallel, then
+    // every orientation lane sums them in FP64.  Staging once per queue rather
+    // than once per chunk keeps the FP64 code off most chunks.
+    int n_near = 0;
+    auto flush_near = [&]() {
+      wave_sync();
+      if (lane < n_near) {
+        const long long id = beg + L.near_idx[wv][lane];
+        const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
+                     qz = list64[3 * id + 2] - cz;
+        const double RR = qx * qx + qy * qy + qz * qz;
+        L.near[wv][lane] = make_double4(qx, qy, qz, RR);
+        if (do_site) S += pair64_d2(RR, rc2, c12, c6, sh, fl2);
+      }
+      wave_sync();
+      for (int q = 0; q < n_near; ++q) {
+        // FP64 near-contact pair: one dot product serves both sites.
+        const double4 d = L.near[wv][q];
+        const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
+        E += pair64_d2(fma(tt, m2o0d, d.w + o0sqd), rc2, c12, c6, sh, fl2);
+        E += pair64_d2(fma(tt, m2o1d, d.w + o1sqd), rc2, c12, c6, sh, fl2);
+      }
+      wave_sync();
+      n_near = 0;
+    };
     for (long long base = beg; base < end; base += 64) {
       const long long idx = base + lane;
       const bool valid = idx < end;
@@ -443,27 +471,21 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, rr + o0sq);
         if (!SYM) L.far1[wv][lanes_below(mfar)] = rr + o1sq;
       }
-      // Near-contact atoms: the compacting lane forms the FP64 center-relative
-      // vector once and stages it in LDS for every orientation lane.
-      double RRn = 0.0;
-      if (is_near) {
-        const double qx = list64[3 * idx] - cx, qy = list64[3 * idx + 1] - cy,
-                     qz = list64[3 * idx + 2] - cz;
-        RRn = qx * qx + qy * qy + qz * qz;
-        L.near[wv][lanes_below(mnear)] = make_double4(qx, qy, qz, RRn);
-      }
+      const int nn = __popcll(mnear);
+      if (n_near + nn > 64) flush_near();
+      if (is_near) L.near_idx[wv][n_near + lanes_below(mnear)] = (int)(idx - beg);
+      n_near += nn;
       if (do_site) {
-        // far candidates: clamped FP32 term (the shift is added per kept entry below);
-        // near candidates: the FP64 reference term from the FP64 offsets.
+        // far candidates: clamped FP32 term (the shift is added per kept entry
+        // below); near candidates get the FP64 reference term when flushed.
         const float ds = __builtin_amdgcn_fmed3f(rr, fl2f, rc2f);
         const float is = __builtin_amdgcn_rcpf(ds);
         const float is6 = is * is * is;
         const float sf = is6 * fmaf(is6, s6, -1.0f);
         Sf += is_far || is_in ? sf : 0.0f;
-        if (is_near) S += pair64_d2(RRn, rc2, c12, c6, sh, fl2);
       }
       wave_sync();
-      const int ni = __popcll(min_), nf = __popcll(mfar), nn = __popcll(mnear);
+      const int ni = __popcll(min_), nf = __popcll(mfar);
       kept_far += ni + nf;
       // FP32 partials (one per site) cover at most kFlush atoms before they are
       // added into the FP64 accumulator, so a partial never grows past a few
@@ -508,15 +530,9 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         E = fma(c6, (double)acc, E);
         E = fma(c6, (double)acc1, E);
       }
-      for (int q = 0; q < nn; ++q) {
-        // FP64 near-contact pair: one dot product serves both sites.
-        const double4 d = L.near[wv][q];
-        const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
-        E += pair64_d2(fma(tt, m2o0d, d.w + o0sqd), rc2, c12, c6, sh, fl2);
-        E += pair64_d2(fma(tt, m2o1d, d.w + o1sqd), rc2, c12, c6, sh, fl2);
-      }
       wave_sync();
     }
+    if (n_near) flush_near();
     // Every kept far pair was clamped to rc^2 at worst, contributing -shift.
     E = fma((double)(2 * kept_far), sh, E);
     S_shift = fma((double)kept_far, sh, S_shift);
