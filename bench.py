@@ -374,9 +374,9 @@ def main():
                          "PMC byte counters can be scaled against a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every pose is independent, "
-              "so N GPUs would each evaluate a slab of centers -- see the sharding note in the "
-              "design document.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): every pose is independent, "
+              "so N GPUs would each evaluate their own slab of centers with no communication; "
+              "this benchmark measures one GPU's share at the full field shape.", file=sys.stderr)
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")

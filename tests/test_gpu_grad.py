@@ -59,7 +59,7 @@ def test_pose_energy_and_gradient_mixed_within_tolerance(oracle, case, poses):
 
 
 def test_gradient_matches_central_differences(case, poses):
-    """h = 1e-4 A on the GPU FP64 energies (config 4's check)."""
+    """Central differences with h = 1e-4 A of the GPU FP64 energies."""
     from string_vext_amd.field import PoseList
     centers, axes = poses
     centers, axes = centers[:120], axes[:120]
