@@ -127,6 +127,38 @@ def cpu_baseline(case, centers, axes, sample):
             "query_seconds": query, "compute_seconds": compute}
 
 
+def cpu_baseline_threads(case, centers, axes, sample, threads):
+    """The same kernel on all host cores: centers split into 64-center pieces run
+    by `threads` Python threads (ctypes releases the GIL during each call); the
+    CSR for every piece is built first and is not part of the timed region."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import vext_oracle as O
+    kind = "reference" if O.reference_kernel() is not None else "port"
+    fn = O.reference_kernel() if kind == "reference" else O.lib().oracle_reuse
+    setup = O.ReuseSetup(case)
+    idx = np.linspace(0, len(centers) - 1, sample).round().astype(int)
+    cart = np.mod(centers[idx], 1) @ setup.cell
+    unit = axes / np.linalg.norm(axes, axis=1, keepdims=True)
+    pieces = []
+    pairs = 0
+    for start in range(0, len(cart), 64):
+        chunk = np.ascontiguousarray(cart[start:start + 64])
+        starts, indices = O.csr(setup.tree, chunk, setup.radius)
+        pieces.append((chunk, starts, indices))
+        pairs += O.lib().oracle_count_pairs(
+            len(chunk), len(unit), setup.cutoff ** 2, O._ptr(chunk), O._ptr(np.ascontiguousarray(unit)),
+            O._ptr(np.ascontiguousarray(setup.images)), O._ptr(np.ascontiguousarray(setup.offsets)),
+            O._ptr(starts), O._ptr(indices))
+    t = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        list(pool.map(lambda p: O.run_reuse_kernel(fn, p[0], unit, setup, p[1], p[2]), pieces))
+    wall = time.perf_counter() - t
+    return {"value": pairs / wall, "unit": "pair-evals/s", "cores": threads, "kind": kind,
+            "sample": f"{sample} grid centers x {len(axes)} orientations in 64-center pieces on "
+                      f"{threads} threads, {pairs} in-cutoff pairs, kernel wall time {wall:.2f} s; "
+                      f"{cpu_model()}"}
+
+
 def gpu_error_metrics(ref, cand):
     """error_metrics on device tensors (accessible: E < 2000 K and |E| < 1e4 K)."""
     import torch
@@ -419,11 +451,14 @@ def main():
     if prof.exists():
         traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
 
-    cpu = None
+    cpu = cpu_all = None
     if args.cpu_sample > 0:
         log(f"cpu baseline on {args.cpu_sample} centers ...")
         cpu = cpu_baseline(case, centers, axes, args.cpu_sample)
         log(f"cpu: {cpu['value']:.3e} pair-evals/s")
+        threads = max(1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        cpu_all = cpu_baseline_threads(case, centers, axes, 4 * args.cpu_sample, threads)
+        log(f"cpu, {threads} threads: {cpu_all['value']:.3e} pair-evals/s")
 
     if args.copy_calibration > 0:
         import torch
@@ -451,6 +486,7 @@ def main():
                      "kernel": "field_mixed_kernel", "kernel_ms": field_ms,
                      "flop_per_pair": FLOP_PER_PAIR},
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline_all_cores": cpu_all,
         "fp64": {"value": pairs / step64, "ms_per_step": step64 * 1e3, "kernel_ms": field64,
                  "frac_of_fp64_peak": pairs * FLOP_PER_PAIR / (field64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
         "speedup_mixed_vs_fp64": step64 / step_s,
