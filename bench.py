@@ -96,9 +96,10 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(case, centers, axes, sample):
+def cpu_baseline(case, centers, axes, sample, keep_output=False):
     """Reference kernel.cpp (built from source into oracle/_ref) or, if absent, the
-    oracle's C restatement, single-threaded on `sample` grid centers."""
+    oracle's C restatement, single-threaded on `sample` grid centers (with
+    keep_output, their energies are returned under "energy")."""
     from oracle import vext_oracle as O
     kind = "reference" if O.reference_kernel() is not None else "port"
     fn = O.reference_kernel() if kind == "reference" else O.lib().oracle_reuse
@@ -108,23 +109,59 @@ def cpu_baseline(case, centers, axes, sample):
     unit = axes / np.linalg.norm(axes, axis=1, keepdims=True)
     query = compute = 0.0
     pairs = 0
+    out = []
     for start in range(0, len(cart), 512):
         chunk = np.ascontiguousarray(cart[start:start + 512])
         t = time.perf_counter()
         starts, indices = O.csr(setup.tree, chunk, setup.radius)
         query += time.perf_counter() - t
         t = time.perf_counter()
-        O.run_reuse_kernel(fn, chunk, unit, setup, starts, indices)
+        e = O.run_reuse_kernel(fn, chunk, unit, setup, starts, indices)
         compute += time.perf_counter() - t
+        if keep_output:
+            out.append(e)
         pairs += O.lib().oracle_count_pairs(
             len(chunk), len(unit), setup.cutoff ** 2, O._ptr(chunk), O._ptr(np.ascontiguousarray(unit)),
             O._ptr(np.ascontiguousarray(setup.images)), O._ptr(np.ascontiguousarray(setup.offsets)),
             O._ptr(starts), O._ptr(indices))
-    return {"value": pairs / compute, "unit": "pair-evals/s", "cores": 1, "kind": kind,
-            "sample": f"{sample} of the {len(centers)} grid centers (evenly spaced) x {len(axes)} "
-                      f"orientations, {pairs} in-cutoff pairs; kernel {compute:.2f} s, cKDTree "
-                      f"candidate query {query:.2f} s; {cpu_model()}",
-            "query_seconds": query, "compute_seconds": compute}
+    res = {"value": pairs / compute, "unit": "pair-evals/s", "cores": 1, "kind": kind,
+           "sample": f"{sample} of the {len(centers)} grid centers (evenly spaced) x {len(axes)} "
+                     f"orientations, {pairs} in-cutoff pairs; kernel {compute:.2f} s, cKDTree "
+                     f"candidate query {query:.2f} s; {cpu_model()}",
+           "query_seconds": query, "compute_seconds": compute}
+    if keep_output:
+        res["energy"] = np.concatenate(out)
+    return res
+
+
+def small_field_report(threads):
+    """The 20^3 x 64 field of the 300-atom framework (25 x 25 x 12 A, gamma 120): GPU
+    mixed and FP64 against the reference CPU kernel's whole field on the same inputs,
+    and that kernel's single-core and all-core rates."""
+    from string_vext_amd import synthetic
+    from string_vext_amd.field import Plan
+    from string_vext_amd.orientation import regular_pose_grid
+    case = synthetic.small_field_case()
+    centers, axes = regular_pose_grid(20, 64)
+    plan = Plan.from_case(case)
+    poses = plan.grid(20, axes)
+    pairs = plan.census(poses)
+    s_mx, k_mx, _, _ = time_field(plan, poses, "mixed", 20, 3, case["temperature"])
+    s_64, k_64, _, _ = time_field(plan, poses, "fp64", 10, 2, case["temperature"])
+    cpu1 = cpu_baseline(case, centers, axes, len(centers), keep_output=True)
+    cpun = cpu_baseline_threads(case, centers, axes, len(centers), threads)
+    ref = cpu1.pop("energy")
+    e64 = energies(plan, poses, "fp64")
+    emx = energies(plan, poses, "mixed")
+    rel64 = np.abs(e64 - ref) / np.maximum(np.abs(ref), 1.0)
+    return {"workload": "20^3 x 64 field, 300-atom framework 25x25x12 A gamma 120, cutoff 12.8 A",
+            "pair_evals": int(pairs),
+            "gpu_mixed": {"value": pairs / s_mx, "ms_per_field": s_mx * 1e3, "kernel_ms": k_mx},
+            "gpu_fp64": {"value": pairs / s_64, "ms_per_field": s_64 * 1e3, "kernel_ms": k_64},
+            "cpu_1_core": {k: cpu1[k] for k in ("value", "cores", "kind", "sample")},
+            "cpu_all_cores": cpun,
+            "fp64_vs_reference_cpu_max_relative_error_floor1K": float(rel64.max()),
+            "mixed_vs_reference_cpu": error_metrics(ref, emx)}
 
 
 def cpu_baseline_threads(case, centers, axes, sample, threads):
@@ -451,7 +488,7 @@ def main():
     if prof.exists():
         traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
 
-    cpu = cpu_all = None
+    cpu = cpu_all = small = None
     if args.cpu_sample > 0:
         log(f"cpu baseline on {args.cpu_sample} centers ...")
         cpu = cpu_baseline(case, centers, axes, args.cpu_sample)
@@ -459,6 +496,8 @@ def main():
         threads = max(1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         cpu_all = cpu_baseline_threads(case, centers, axes, 4 * args.cpu_sample, threads)
         log(f"cpu, {threads} threads: {cpu_all['value']:.3e} pair-evals/s")
+        small = small_field_report(threads)
+        log(f"20^3 field: {small}")
 
     if args.copy_calibration > 0:
         import torch
@@ -487,6 +526,7 @@ def main():
                      "flop_per_pair": FLOP_PER_PAIR},
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "cpu_baseline_all_cores": cpu_all,
+        "small_field": small,
         "fp64": {"value": pairs / step64, "ms_per_step": step64 * 1e3, "kernel_ms": field64,
                  "frac_of_fp64_peak": pairs * FLOP_PER_PAIR / (field64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
         "speedup_mixed_vs_fp64": step64 / step_s,
