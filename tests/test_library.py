@@ -53,3 +53,36 @@ def test_dropin_validates_sizes_before_touching_the_device():
     assert lib.evaluate_reused_neighbors(1, 4, 0, 1.0, *([null] * 11)) == 1
     assert lib.evaluate_reused_neighbors(1, 4, 1, 0.0, *([null] * 11)) == 1
     assert lib.evaluate_reused_neighbors(0, 4, 1, 1.0, *([null] * 11)) == 0
+
+
+def test_argument_validation_codes_on_cpu_host():
+    """Invalid arguments are rejected with code 1 before any device work; a valid
+    plan on a host without a GPU fails with code 5 (HIP runtime)."""
+    import numpy as np
+    from string_vext_amd import _lib
+    lib = _lib.load(require_device=False)
+    P = ctypes.c_void_p
+
+    def ptr(a):
+        return a.ctypes.data_as(P)
+
+    frac = np.zeros((1, 3))
+    sig, eps, off = np.array([3.0]), np.array([10.0]), np.array([-0.77, 0.77])
+    h = P()
+
+    def plan(cell, n=1, sigma=sig, epsilon=eps, cutoff=12.0):
+        cell = np.ascontiguousarray(cell, dtype=np.float64).ravel()
+        return lib.sv_plan_create(ptr(cell), n, ptr(frac), ptr(sigma), ptr(epsilon), 3.0, 100.0,
+                                  ptr(off), cutoff, 50.0, ctypes.byref(h))
+
+    assert plan(np.diag([10.0, 10.0, -10.0])) == 1            # det <= 0
+    assert plan(np.eye(3) * 10, n=0) == 1
+    assert plan(np.eye(3) * 10, sigma=-sig) == 1
+    assert plan(np.eye(3) * 10, epsilon=np.array([np.nan])) == 1
+    assert plan(np.eye(3) * 10, cutoff=0.0) == 1
+    assert plan(np.eye(3) * 10) == 5                           # no device here
+    assert lib.sv_pose_grid_eval(None, 1, None, None, 0, None, None, None, None) == 1
+    assert lib.sv_pose_grid_check(None) == 1
+    assert lib.sv_strings_create(0, None, 5, None, None, 1e-4, 0.01, 0, ctypes.byref(h)) == 1
+    assert lib.sv_batch_run(None, 0, 1) == 1
+    assert b"null" in lib.sv_last_error() or lib.sv_last_error()
