@@ -18,7 +18,9 @@
 //     both sites.
 //
 // Mixed precision: the pair arithmetic is FP32 with the distance clamped to
-// [floor2, rc^2] by one v_med3_f32.  Because the shifted potential vanishes at
+// at most rc^2 (for shell atoms d^2/rc^2 is clamped to [0, 1] by the clamp bit
+// of the fma that forms it; atoms an orientation could bring within the
+// contact floor are always FP64).  Because the shifted potential vanishes at
 // rc, a clamped out-of-range pair contributes exactly -shift, which the
 // per-type count of kept pairs adds back in FP64; no cutoff branch or select
 // is needed and a pair on the wrong side of rc by FP32 rounding changes the
@@ -405,6 +407,9 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
   const float o0sq = J.o0sqf, o1sq = J.o1sqf, in2 = J.in2f;
   const double rc2 = J.rc2;
+  const double rc6 = rc2 * rc2 * rc2;
+  const float inv_rc2f = (float)(1.0 / rc2);
+  const float m2o0s = (float)(-2.0 * J.o0 / rc2), m2o1s = (float)(-2.0 * J.o1 / rc2);
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
   const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1;
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
@@ -414,6 +419,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t], lo2 = J.lo2f[t];
     const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
+    const double c6s = c6 / rc6;                      // shell loop scale
+    const float s6s = (float)(c12 / c6 / rc6);
     long long kept_far = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
     // Near-contact atoms are queued by list offset and evaluated when the queue
@@ -453,10 +460,12 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         rr = rx * rx + ry * ry + rz * rz;
       }
       const bool keep = rr < keep2f;
-      const bool is_near = keep && rr < near2;
-      // inner: every orientation puts both sites strictly between the contact
-      // floor and the cutoff, so the pair needs no clamp
-      const bool is_in = keep && !is_near && rr < in2 && rr > lo2;
+      // Atoms some orientation brings within the contact floor are always FP64,
+      // so the FP32 classes never need the floor clamp.
+      const bool is_near = keep && (rr < near2 || rr <= lo2);
+      // inner: every orientation puts both sites strictly inside the cutoff, so
+      // the pair needs no clamp
+      const bool is_in = keep && !is_near && rr < in2;
       const bool is_far = keep && !is_near && !is_in;
       const unsigned long long min_ = __ballot(is_in);
       const unsigned long long mfar = __ballot(is_far);
@@ -468,8 +477,9 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         if (!SYM) L.in1[wv][lanes_below(min_)] = rr + o1sq;
       }
       if (is_far) {
-        s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, rr + o0sq);
-        if (!SYM) L.far1[wv][lanes_below(mfar)] = rr + o1sq;
+        // shell entries in units of rc^2 (see the shell loop)
+        s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, (rr + o0sq) * inv_rc2f);
+        if (!SYM) L.far1[wv][lanes_below(mfar)] = (rr + o1sq) * inv_rc2f;
       }
       const int nn = __popcll(mnear);
       if (n_near + nn > 64) flush_near();
@@ -509,6 +519,10 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         E = fma(c6, (double)acc, E);
         E = fma(c6, (double)acc1, E);
       }
+      // Shell atoms work in units of rc^2: x = d^2 / rc^2 is clamped to [0, 1]
+      // by the clamp bit of the fma that forms it (med3(., 0, 1) folds into it),
+      // and c6 / rc^6, sigma^6 / rc^6 absorb the scale.  A pair beyond the cutoff
+      // gives x = 1, i.e. exactly the clamped -shift of the unscaled form.
       for (int q0 = 0; q0 < nf; q0 += kFlush) {
         const int q1 = q0 + kFlush < nf ? q0 + kFlush : nf;
         float acc = 0.f, acc1 = 0.f;
@@ -516,19 +530,18 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         for (int q = q0; q < q1; ++q) {
           const float4 f = s_far[wv][q];
           const float tdot = f.x * ux + f.y * uy + f.z * uz;
-          float d0 = fmaf(tdot, m2o0, f.w);
-          float d1 = fmaf(tdot, m2o1, SYM ? f.w : L.far1[wv][q]);
-          d0 = __builtin_amdgcn_fmed3f(d0, fl2f, rc2f);
-          d1 = __builtin_amdgcn_fmed3f(d1, fl2f, rc2f);
-          const float i0 = __builtin_amdgcn_rcpf(d0);
-          const float i1 = __builtin_amdgcn_rcpf(d1);
+          const float x0 = __builtin_amdgcn_fmed3f(fmaf(tdot, m2o0s, f.w), 0.0f, 1.0f);
+          const float x1 =
+              __builtin_amdgcn_fmed3f(fmaf(tdot, m2o1s, SYM ? f.w : L.far1[wv][q]), 0.0f, 1.0f);
+          const float i0 = __builtin_amdgcn_rcpf(x0);
+          const float i1 = __builtin_amdgcn_rcpf(x1);
           const float i06 = i0 * i0 * i0;
           const float i16 = i1 * i1 * i1;
-          acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
-          acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
+          acc = fmaf(i06, fmaf(i06, s6s, -1.0f), acc);
+          acc1 = fmaf(i16, fmaf(i16, s6s, -1.0f), acc1);
         }
-        E = fma(c6, (double)acc, E);
-        E = fma(c6, (double)acc1, E);
+        E = fma(c6s, (double)acc, E);
+        E = fma(c6s, (double)acc1, E);
       }
       wave_sync();
     }
