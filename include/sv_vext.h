@@ -69,7 +69,8 @@ int sv_device_count(void);
 /*                      (physics.py:88-90)                                   */
 /*   near_energy_K      bound on a single FP32 pair term in the mixed        */
 /*                      precision kernel; closer atoms are evaluated in FP64. */
-/*                      <= 0 disables the FP64 near path.                    */
+/*                      <= 0 keeps FP64 only for atoms within reach of the  */
+/*                      contact floor (0.1 sigma + max|offset|).             */
 /* ------------------------------------------------------------------------ */
 int sv_plan_create(const double *cell, int n_atoms, const double *frac, const double *sigma,
                    const double *epsilon, double guest_sigma, double guest_epsilon,
