@@ -37,6 +37,8 @@ struct Job {
   float o1sqf;          // o1^2
   float in2f;           // (rc - max|o| - margin)^2: closer atoms are inside the
                         // cutoff for every orientation
+  float inv_rc2f;       // 1/rc^2: shell atoms of the mixed path work in units of rc^2
+  float m2o0sf, m2o1sf; // -2*o0/rc^2, -2*o1/rc^2
   int sym;              // o1 == -o0 exactly
   // ---- per-type coefficients (index = framework LJ type) ------------------
   const double* c12;    // 4 eps sigma^12          (mixed Lorentz-Berthelot)
@@ -51,6 +53,8 @@ struct Job {
                         // mixed kernel evaluates the pair in FP64
   const float* lo2f;    // (sqrt(floor2) + max|o| + margin)^2: beyond it no
                         // orientation reaches the contact floor
+  const double* c6s;    // c6 / rc^6            (shell form of the mixed path)
+  const float* s6sf;    // sigma^6 / rc^6 in FP32
   // ---- pose set -----------------------------------------------------------
   const double* center_rel;  // [n_centers][3] Cartesian offset from tile centre (sorted)
   const int* center_id;      // sorted slot -> caller's center index

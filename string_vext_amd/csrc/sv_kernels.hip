@@ -407,9 +407,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
   const float o0sq = J.o0sqf, o1sq = J.o1sqf, in2 = J.in2f;
   const double rc2 = J.rc2;
-  const double rc6 = rc2 * rc2 * rc2;
-  const float inv_rc2f = (float)(1.0 / rc2);
-  const float m2o0s = (float)(-2.0 * J.o0 / rc2), m2o1s = (float)(-2.0 * J.o1 / rc2);
+  const float inv_rc2f = J.inv_rc2f, m2o0s = J.m2o0sf, m2o1s = J.m2o1sf;
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
   const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1;
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
@@ -419,8 +417,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t], lo2 = J.lo2f[t];
     const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
-    const double c6s = c6 / rc6;                      // shell loop scale
-    const float s6s = (float)(c12 / c6 / rc6);
+    const double c6s = J.c6s[t];   // shell loop scale
+    const float s6s = J.s6sf[t];
     long long kept_far = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
     // Near-contact atoms are queued by list offset and evaluated when the queue

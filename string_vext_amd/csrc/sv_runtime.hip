@@ -95,6 +95,11 @@ void plan_fill_job(const Plan& P, Job& J) {
   J.floor2f = P.d_floor2f.p;
   J.near2f = P.d_near2f.p;
   J.lo2f = P.d_lo2f.p;
+  J.c6s = P.d_c6s.p;
+  J.s6sf = P.d_s6sf.p;
+  J.inv_rc2f = (float)(1.0 / J.rc2);
+  J.m2o0sf = (float)(-2.0 * P.o0 / J.rc2);
+  J.m2o1sf = (float)(-2.0 * P.o1 / J.rc2);
 }
 
 namespace {
@@ -541,6 +546,10 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
     P->eps.push_back(me);
     P->s6f.push_back((float)(c12 / c6));
     P->floor2f.push_back((float)fl);
+    // coefficients of the shell form, in units of rc^2
+    const double rc6 = cutoff * cutoff * cutoff * cutoff * cutoff * cutoff;
+    P->c6s.push_back(c6 / rc6);
+    P->s6sf.push_back((float)(c12 / c6 / rc6));
     float near2 = 0.0f;
     if (near_energy > 0.0) {
       // Repulsive-branch distance where 4 eps (x^2 - x) = near_energy, x = (sigma/r)^6.
@@ -577,6 +586,8 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
       (e = P->d_floor2f.upload(P->floor2f.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = P->d_near2f.upload(P->near2f.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = P->d_lo2f.upload(P->lo2f.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_c6s.upload(P->c6s.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_s6sf.upload(P->s6sf.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
     delete P;
     return hip_fail(e, "plan upload");
