@@ -115,7 +115,10 @@ struct SiteChunk {
 // ---- kernel launchers (sv_kernels.hip) ------------------------------------
 hipError_t launch_tile_count(const Job* d_jobs, const Tile* d_tiles, int n_tiles_total,
                              const long long* d_tile_seg_base, int* d_counts, hipStream_t s);
-hipError_t launch_scan(const int* d_counts, long long n, long long* d_offsets, hipStream_t s);
+// Exclusive scan; d_sums holds scan_scratch_size(n) entries of scratch.
+long long scan_scratch_size(long long n);
+hipError_t launch_scan(const int* d_counts, long long n, long long* d_offsets, long long* d_sums,
+                       hipStream_t s);
 hipError_t launch_tile_fill(const Job* d_jobs, const Tile* d_tiles, int n_tiles_total,
                             const long long* d_tile_seg_base, const long long* d_offsets,
                             float4* d_list32, double* d_list64, hipStream_t s);

@@ -160,30 +160,93 @@ __global__ __launch_bounds__(64) void tile_count_kernel(const Job* __restrict__ 
   }
 }
 
-// Exclusive scan of counts[0..n) into offsets[0..n]; one block of 1024 threads.
-__global__ __launch_bounds__(1024) void scan_kernel(const int* __restrict__ counts, long long n,
-                                                    long long* __restrict__ offsets) {
-  __shared__ long long s_part[1024];
-  const int tid = threadIdx.x;
-  const long long per = (n + 1023) / 1024;
-  const long long beg = tid * per < n ? tid * per : n;
-  const long long end = beg + per < n ? beg + per : n;
+// Exclusive scan of counts[0..n) into offsets[0..n], in three launches: every
+// block sums one chunk of kScanChunk counts, one block scans the chunk sums,
+// then every block scans its chunk from its chunk's offset.  Thread i of a
+// block owns the kScanItems consecutive counts starting at i * kScanItems.
+constexpr int kScanThreads = 1024, kScanItems = 8, kScanChunk = kScanThreads * kScanItems;
+
+__device__ __forceinline__ long long scan_thread_items(const int* __restrict__ counts, long long n,
+                                                       long long first, int v[kScanItems]) {
   long long sum = 0;
-  for (long long i = beg; i < end; ++i) sum += counts[i];
-  s_part[tid] = sum;
+  if (first + kScanItems <= n) {
+    const int4 a = *reinterpret_cast<const int4*>(counts + first);
+    const int4 b = *reinterpret_cast<const int4*>(counts + first + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) v[j] = first + j < n ? counts[first + j] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) sum += v[j];
+  return sum;
+}
+
+// Exclusive prefix of `x` over the block's threads; `total` gets the block sum.
+__device__ __forceinline__ long long block_exclusive_scan(long long x, long long* total) {
+  __shared__ long long s_w[kScanThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  long long inc = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) s_w[w] = inc;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const long long v = tid >= o ? s_part[tid - o] : 0;
-    __syncthreads();
-    s_part[tid] += v;
-    __syncthreads();
+  long long before = 0, all = 0;
+  for (int k = 0; k < kScanThreads / 64; ++k) {
+    before += k < w ? s_w[k] : 0;
+    all += s_w[k];
   }
-  long long run = s_part[tid] - sum;
+  *total = all;
+  return before + inc - x;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_sums_kernel(const int* __restrict__ counts,
+                                                                 long long n,
+                                                                 long long* __restrict__ sums) {
+  int v[kScanItems];
+  const long long first = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
+  long long total;
+  (void)block_exclusive_scan(scan_thread_items(counts, n, first, v), &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the nb chunk sums in place; offsets[n] = total.
+__global__ __launch_bounds__(kScanThreads) void scan_chunks_kernel(long long* __restrict__ sums,
+                                                                   long long nb, long long n,
+                                                                   long long* __restrict__ offsets) {
+  const long long per = (nb + kScanThreads - 1) / kScanThreads;
+  const long long beg = threadIdx.x * per < nb ? threadIdx.x * per : nb;
+  const long long end = beg + per < nb ? beg + per : nb;
+  long long mine = 0;
+  for (long long i = beg; i < end; ++i) mine += sums[i];
+  long long total;
+  long long run = block_exclusive_scan(mine, &total);
   for (long long i = beg; i < end; ++i) {
-    offsets[i] = run;
-    run += counts[i];
+    const long long c = sums[i];
+    sums[i] = run;
+    run += c;
   }
-  if (tid == 1023) offsets[n] = s_part[1023];
+  if (threadIdx.x == 0) offsets[n] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const int* __restrict__ counts,
+                                                                  long long n,
+                                                                  const long long* __restrict__ sums,
+                                                                  long long* __restrict__ offsets) {
+  int v[kScanItems];
+  const long long first = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
+  long long total;
+  long long run = sums[blockIdx.x] +
+                  block_exclusive_scan(scan_thread_items(counts, n, first, v), &total);
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if (first + j < n) offsets[first + j] = run;
+    run += v[j];
+  }
 }
 
 __global__ __launch_bounds__(64) void tile_fill_kernel(const Job* __restrict__ jobs,
@@ -1120,8 +1183,17 @@ hipError_t launch_tile_count(const Job* d_jobs, const Tile* d_tiles, int n_tiles
   return hipGetLastError();
 }
 
-hipError_t launch_scan(const int* d_counts, long long n, long long* d_offsets, hipStream_t s) {
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, d_counts, n, d_offsets);
+long long scan_scratch_size(long long n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
+
+hipError_t launch_scan(const int* d_counts, long long n, long long* d_offsets, long long* d_sums,
+                       hipStream_t s) {
+  const long long nb = (n + kScanChunk - 1) / kScanChunk;
+  if (nb > 0) hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s,
+                                 d_counts, n, d_sums);
+  hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads), 0, s, d_sums, nb, n,
+                     d_offsets);
+  if (nb > 0) hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s,
+                                 d_counts, n, d_sums, d_offsets);
   return hipGetLastError();
 }
 

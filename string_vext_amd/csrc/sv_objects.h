@@ -90,7 +90,7 @@ struct Batch {
   std::vector<SiteChunk> chunks;
   long long n_segs = 0, n_work = 0, n_centers_total = 0, list_total = -1;
   DevBuf<Job> d_jobs;
-  DevBuf<long long> d_job_work, d_job_first, d_seg_base, d_offsets;
+  DevBuf<long long> d_job_work, d_job_first, d_seg_base, d_offsets, d_scan;
   DevBuf<Tile> d_tiles;
   DevBuf<SiteChunk> d_chunks;
   DevBuf<int> d_counts, d_bad;

@@ -313,6 +313,7 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
       (e = b->d_chunks.upload(b->chunks.data(), b->chunks.size(), b->stream)) != hipSuccess ||
       (e = b->d_counts.reserve(std::max<long long>(segs, 1))) != hipSuccess ||
       (e = b->d_offsets.reserve(segs + 1)) != hipSuccess ||
+      (e = b->d_scan.reserve(scan_scratch_size(segs))) != hipSuccess ||
       (e = b->d_bad.reserve(1)) != hipSuccess ||
       (e = b->d_pairs.reserve(n_jobs)) != hipSuccess) {
     delete B;
@@ -347,7 +348,7 @@ int batch_lists(Batch* B, bool timed) {
   if (timed) SV_HIP(hipEventRecord(B->ev[0], B->stream));
   SV_HIP(launch_tile_count(B->d_jobs.p, B->d_tiles.p, nt, B->d_seg_base.p, B->d_counts.p, B->stream));
   if (timed) SV_HIP(hipEventRecord(B->ev[1], B->stream));
-  SV_HIP(launch_scan(B->d_counts.p, B->n_segs, B->d_offsets.p, B->stream));
+  SV_HIP(launch_scan(B->d_counts.p, B->n_segs, B->d_offsets.p, B->d_scan.p, B->stream));
   if (timed) SV_HIP(hipEventRecord(B->ev[2], B->stream));
   if (B->list_total < 0) {
     long long total = 0;
