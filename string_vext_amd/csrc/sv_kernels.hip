@@ -699,8 +699,10 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
 
 // ---------------------------------------------------------------------------
 // Independent poses (center_i, director_i): energy and, with GRAD, the
-// analytic gradient.  Lanes are up to 64 poses of one tile; every lane walks
-// the tile's whole candidate list (read by broadcast from LDS).  With
+// analytic gradient.  Lanes are up to 64 poses of one tile; the block's
+// kPoseWaves waves split the tile's candidate list (64-entry pieces dealt
+// round-robin, read by broadcast from LDS) and their per-lane sums are added
+// in wave order at the end, so one chunk of poses fills several SIMDs.  With
 // r = image - center, t = r.u and g_s = dphi/d(d^2) of site s,
 //   dE/dc = 2 [u sum_s o_s G_s - sum_j r_j (g_0j + g_1j)]
 //   dE/du = 2 [u sum_s o_s^2 G_s - sum_j r_j (o_0 g_0j + o_1 g_1j)]
@@ -710,6 +712,8 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
 struct PoseAcc {
   double E, G0, G1, Rc[3], Ru[3], Gc[3], Gu[3];
 };
+constexpr int kPoseWaves = 4;
+constexpr int kPoseAccN = 15;   // doubles in a PoseAcc (E, G0, G1, Rc, Ru, Gc, Gu)
 
 // FP64 pair energy and gradient contribution with explicit d = o u - r.
 __device__ __forceinline__ void pose_pair64(double o, double ux, double uy, double uz, double rx,
@@ -733,20 +737,24 @@ __device__ __forceinline__ void pose_pair64(double o, double ux, double uy, doub
 }
 
 template <bool MIXED, bool GRAD>
-__global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
-                                                  const SiteChunk* __restrict__ chunks,
-                                                  const int* __restrict__ n_chunks,
-                                                  const long long* __restrict__ offsets,
-                                                  const float4* __restrict__ list32,
-                                                  const double* __restrict__ list64) {
-  __shared__ float4 s_e[64];
-  __shared__ double s_d[3][64];
+__global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __restrict__ jobs,
+                                                               const SiteChunk* __restrict__ chunks,
+                                                               const int* __restrict__ n_chunks,
+                                                               const long long* __restrict__ offsets,
+                                                               const float4* __restrict__ list32,
+                                                               const double* __restrict__ list64) {
+  __shared__ float4 s_e_all[kPoseWaves][64];
+  __shared__ double s_d_all[kPoseWaves][3][64];
+  __shared__ double s_acc[kPoseWaves - 1][GRAD ? kPoseAccN : 1][64];
   // A chunk table built on the device is launched with a grid that bounds its
   // length; blocks past the actual count have nothing to do.
   if (n_chunks && (int)blockIdx.x >= *n_chunks) return;
   const SiteChunk ch = chunks[blockIdx.x];
   const Job& J = jobs[ch.job];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wv = uniform(threadIdx.x >> 6);
+  float4* s_e = s_e_all[wv];
+  double (*s_d)[64] = s_d_all[wv];
   const bool ok = lane < ch.count;
   const int slot = ch.first + (ok ? lane : 0);
   const double cx = J.center_rel[3 * slot], cy = J.center_rel[3 * slot + 1],
@@ -766,7 +774,7 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
     const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
     int n_in = 0;   // in-cutoff FP32 pairs of this type
-    for (long long base = beg; base < end; base += 64) {
+    for (long long base = beg + 64 * wv; base < end; base += 64 * kPoseWaves) {
       const long long idx = base + lane;
       if (idx < end) {
         if (MIXED) s_e[lane] = list32[idx];
@@ -845,7 +853,40 @@ __global__ __launch_bounds__(64) void pose_kernel(const Job* __restrict__ jobs,
     }
     if (MIXED) a.E = fma((double)n_in, sh, a.E);
   }
-  if (!ok) return;
+  // Waves 1.. hand their sums to wave 0, which adds them in wave order.
+  if (wv > 0) {
+    double(*o)[64] = s_acc[wv - 1];
+    o[0][lane] = a.E;
+    if (GRAD) {
+      o[1][lane] = a.G0;
+      o[2][lane] = a.G1;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        o[3 + k][lane] = a.Rc[k];
+        o[6 + k][lane] = a.Ru[k];
+        o[9 + k][lane] = a.Gc[k];
+        o[12 + k][lane] = a.Gu[k];
+      }
+    }
+  }
+  __syncthreads();
+  if (wv > 0 || !ok) return;
+#pragma unroll
+  for (int w = 0; w < kPoseWaves - 1; ++w) {
+    const double(*o)[64] = s_acc[w];
+    a.E += o[0][lane];
+    if (GRAD) {
+      a.G0 += o[1][lane];
+      a.G1 += o[2][lane];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        a.Rc[k] += o[3 + k][lane];
+        a.Ru[k] += o[6 + k][lane];
+        a.Gc[k] += o[9 + k][lane];
+        a.Gu[k] += o[12 + k][lane];
+      }
+    }
+  }
   const int cid = J.center_id[slot];
   // A pose with a nonfinite center or a nonfinite / zero director (only device
   // binned poses can carry one; it is flagged there) gets NaN results rather
@@ -1263,7 +1304,7 @@ hipError_t launch_pose(int precision, bool grad, const Job* d_jobs, const SiteCh
                        int n_chunks, const int* d_n_chunks, const long long* d_offsets,
                        const float4* d_list32, const double* d_list64, hipStream_t s) {
   if (n_chunks <= 0) return hipSuccess;
-  const dim3 g(n_chunks), b(64);
+  const dim3 g(n_chunks), b(64 * kPoseWaves);
   if (precision == PREC_MIXED) {
     if (grad)
       hipLaunchKernelGGL((pose_kernel<true, true>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,

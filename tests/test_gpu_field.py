@@ -156,10 +156,10 @@ def test_arbitrary_centers_and_many_axes(oracle, small_case):
 
 @pytest.mark.parametrize("near_energy", [0.0, 5.0, 1000.0])
 def test_mixed_near_contact_bound_settings(oracle, small_case, near_energy):
-    """near_energy_K <= 0 turns the FP64 near-contact path off (every kept pair
-    in FP32, the floor handled by the clamp); other values move the energy
-    bound (the wall-slope bound still applies).  With the path off the
-    accessible error exceeds 1e-3 K (1.1e-3 K here): that is what the FP64
+    """near_energy_K <= 0 keeps only the atoms within reach of the contact
+    floor in FP64 (every other kept pair in FP32); other values move the
+    energy bound (the wall-slope bound still applies).  With the near radius
+    off the accessible error can exceed 1e-3 K: that is what the FP64
     near-contact terms are for."""
     from string_vext_amd.field import Plan
     size, k = 6, 64
@@ -170,3 +170,21 @@ def test_mixed_near_contact_bound_settings(oracle, small_case, near_energy):
     m = oracle.error_metrics(ref, out)
     assert m["accessible_max_abs_error_K"] < (1e-3 if near_energy > 0 else 1e-2)
     assert m["all_max_relative_error_floor1K"] < 1e-4
+
+
+def test_many_list_segments_multi_block_scan(small_case):
+    """Tiles of about one center: 20 x 20 x 10 tiles x 3 types = 12,000
+    (tile, type) list segments, so the offsets scan runs over more than one
+    8192-count chunk.  The field and the pair census must equal those of the
+    default tiling."""
+    from string_vext_amd.field import Plan
+    from string_vext_amd.orientation import regular_pose_grid
+    _, axes = regular_pose_grid(20, 64)
+    plan = Plan.from_case(small_case)
+    fine = plan.grid(20, axes, tile_centers=1)
+    coarse = plan.grid(20, axes)
+    assert plan.census(fine) == plan.census(coarse)
+    a = plan.field(fine, precision="fp64")["energy"]
+    b = plan.field(coarse, precision="fp64")["energy"]
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+    assert rel.max() < 1e-12, rel.max()
