@@ -190,6 +190,13 @@ int sv_pose_grid_eval_async(void *grid, void *stream, int n, const double *cente
                             const double *axes, int precision, const double *grad_out,
                             double *energy, double *grad_center, double *grad_axis);
 int sv_pose_grid_check(void *grid);
+/* sv_pose_grid_reeval_async evaluates the poses of the grid's last            */
+/* sv_pose_grid_eval(_async) call again -- e.g. the gradient pass after an    */
+/* energy pass -- from the binned copies the grid keeps, skipping the         */
+/* binning (the caller's pose arrays are not read).  Outputs, precision and   */
+/* grad_out are this call's.  Returns 1 when no poses have been binned.       */
+int sv_pose_grid_reeval_async(void *grid, void *stream, int precision, const double *grad_out,
+                              double *energy, double *grad_center, double *grad_axis);
 
 /* ------------------------------------------------------------------------ */
 /* Batched String method: n_strings independent strings of n_images images. */

@@ -10,9 +10,10 @@ the device: each call bins them into the plan's cached tile grid on the GPU,
 enqueued on torch's current stream without a host synchronization
 (sv_pose_grid_eval_async); invalid poses give NaN energies, like other torch
 operations, and ``check_poses(plan)`` raises on them.  Forward runs the energy
-kernel; backward runs a separate
-kernel that recomputes every pair term and returns dE/dc and dE/du scaled by
-the upstream gradient.  There is no counterpart in the reference, whose String
+kernel; backward runs a separate kernel that recomputes every pair term and
+returns dE/dc and dE/du scaled by the upstream gradient.  When no other
+evaluation has used the grid since this forward, backward reuses the
+forward's binning (sv_pose_grid_reeval_async).  There is no counterpart in the reference, whose String
 forces are forward differences of the energy
 (native/string/source_code/global.h:1281-1324).
 """
@@ -45,10 +46,12 @@ class PoseEnergy(torch.autograd.Function):
         energy = torch.empty(len(c), dtype=torch.float64, device=device)
         grid = _grid_for(plan, len(c))
         stream = torch.cuda.current_stream(device).cuda_stream
+        token = object()
         grid.eval_async(stream, len(c), c.data_ptr(), u.data_ptr(), precision,
-                        energy=energy.data_ptr())
+                        energy=energy.data_ptr(), token=token)
         ctx.save_for_backward(c, u)
         ctx.grid = grid
+        ctx.token = token
         ctx.precision = precision
         ctx.dtypes = (centers.dtype, axes.dtype)
         return energy
@@ -61,9 +64,15 @@ class PoseEnergy(torch.autograd.Function):
         grad_c = torch.empty((n, 3), dtype=torch.float64, device=c.device)
         grad_u = torch.empty((n, 3), dtype=torch.float64, device=c.device)
         stream = torch.cuda.current_stream(c.device).cuda_stream
-        ctx.grid.eval_async(stream, n, c.data_ptr(), u.data_ptr(), ctx.precision,
-                            grad_out=g.data_ptr(), grad_center=grad_c.data_ptr(),
-                            grad_axis=grad_u.data_ptr())
+        if ctx.grid.binned_token is ctx.token:
+            # the grid still holds this forward's binned poses (saved tensors
+            # cannot have changed: autograd rejects in-place edits of them)
+            ctx.grid.reeval_async(stream, ctx.precision, grad_out=g.data_ptr(),
+                                  grad_center=grad_c.data_ptr(), grad_axis=grad_u.data_ptr())
+        else:
+            ctx.grid.eval_async(stream, n, c.data_ptr(), u.data_ptr(), ctx.precision,
+                                grad_out=g.data_ptr(), grad_center=grad_c.data_ptr(),
+                                grad_axis=grad_u.data_ptr())
         return grad_c.to(ctx.dtypes[0]), grad_u.to(ctx.dtypes[1]), None, None
 
 

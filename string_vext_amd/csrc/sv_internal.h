@@ -151,10 +151,11 @@ hipError_t launch_pose_scatter(int n, const double* d_centers, const double* d_a
                                hipStream_t s);
 hipError_t launch_zero(int* p, int n, hipStream_t s);
 // Sets the per-call pose pointers of job 0 from kernel arguments (stream-ordered).
+// Patches the pose job's arrays and clears zero[0..n_zero) in one launch.
 hipError_t launch_job_patch(Job* d_job, const double* center_rel, const int* center_id,
                             const int* tile_of, const double* axes64, const float* axes32, int n,
                             const double* grad_out, double* pose_e, double* grad_c,
-                            double* grad_u, hipStream_t s);
+                            double* grad_u, int* zero, int n_zero, hipStream_t s);
 hipError_t launch_legacy(int n_centers, int n_axes, double cutoff2, const double* centers,
                          const double* axes, const double* images, const double* offsets,
                          const double* coeff12, const double* coeff6, const double* shift,

@@ -1160,11 +1160,16 @@ __global__ __launch_bounds__(256) void legacy_kernel(int n_centers, int n_axes, 
   }
 }
 
-__global__ void job_patch_kernel(Job* __restrict__ J, const double* center_rel,
+// Points the pose job at this call's arrays and clears `zero[0..n_zero)`
+// (the bin counts of a binning pass, or nothing).
+__global__ __launch_bounds__(256) void job_patch_kernel(Job* __restrict__ J, const double* center_rel,
                                  const int* center_id, const int* tile_of, const double* axes64,
                                  const float* axes32, int n, const double* grad_out,
-                                 double* pose_e, double* grad_c, double* grad_u) {
-  if (threadIdx.x == 0) {
+                                 double* pose_e, double* grad_c, double* grad_u,
+                                 int* __restrict__ zero, int n_zero) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n_zero) zero[i] = 0;
+  if (i == 0) {
     J->center_rel = center_rel;
     J->center_id = center_id;
     J->tile_of = tile_of;
@@ -1204,9 +1209,10 @@ hipError_t launch_zero(int* p, int n, hipStream_t s) {
 hipError_t launch_job_patch(Job* d_job, const double* center_rel, const int* center_id,
                             const int* tile_of, const double* axes64, const float* axes32, int n,
                             const double* grad_out, double* pose_e, double* grad_c,
-                            double* grad_u, hipStream_t s) {
-  hipLaunchKernelGGL(job_patch_kernel, dim3(1), dim3(64), 0, s, d_job, center_rel, center_id,
-                     tile_of, axes64, axes32, n, grad_out, pose_e, grad_c, grad_u);
+                            double* grad_u, int* zero, int n_zero, hipStream_t s) {
+  const int blocks = n_zero > 256 ? (n_zero + 255) / 256 : 1;
+  hipLaunchKernelGGL(job_patch_kernel, dim3(blocks), dim3(256), 0, s, d_job, center_rel, center_id,
+                     tile_of, axes64, axes32, n, grad_out, pose_e, grad_c, grad_u, zero, n_zero);
   return hipGetLastError();
 }
 

@@ -120,6 +120,7 @@ struct PoseGrid {
   int m[3] = {1, 1, 1};
   int n_bins = 0;
   int capacity = 0;          // poses the slot arrays hold
+  int binned_n = 0;          // poses binned by the last enqueue (0: none)
   double radius = 0.0;       // circumscribed radius of one bin
   Mat3 H{}, Hinv{};
   hipStream_t last_stream = nullptr;   // stream of the last enqueued evaluation

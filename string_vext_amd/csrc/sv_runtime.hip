@@ -315,6 +315,9 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
       (e = b->d_offsets.reserve(segs + 1)) != hipSuccess ||
       (e = b->d_scan.reserve(scan_scratch_size(segs))) != hipSuccess ||
       (e = b->d_bad.reserve(1)) != hipSuccess ||
+      // the asynchronous pose-grid calls only ever OR into the flags and
+      // clear them when they are read, so they must start cleared
+      (e = hipMemsetAsync(b->d_bad.p, 0, sizeof(int), b->stream)) != hipSuccess ||
       (e = b->d_pairs.reserve(n_jobs)) != hipSuccess) {
     delete B;
     return hip_fail(e, "batch setup");
@@ -619,7 +622,7 @@ extern "C" {
 
 const char* sv_last_error(void) { return t_error.c_str(); }
 
-int sv_abi_version(void) { return 3; }
+int sv_abi_version(void) { return 4; }
 
 int sv_device_count(void) {
   int n = 0;
@@ -1028,6 +1031,7 @@ int pose_grid_enqueue(PoseGrid* G, hipStream_t st, int n, const double* centers_
   if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
   Batch* B = G->B;
   Poses* S = G->S;
+  G->binned_n = 0;
   SV_HIP(hipSetDevice(G->P->device));
   const long long max_chunks = (long long)G->n_bins + (n + 63) / 64;
   if (max_chunks > INT32_MAX) return fail(1, "too many poses");
@@ -1054,8 +1058,7 @@ int pose_grid_enqueue(PoseGrid* G, hipStream_t st, int n, const double* centers_
   G->last_stream_set = true;
   SV_HIP(launch_job_patch(B->d_jobs.p, S->d_center_rel.p, S->d_center_id.p, S->d_tile_of.p,
                           S->d_pose_axes64.p, S->d_pose_axes32.p, n, grad_out, energy, grad_center,
-                          grad_axis, st));
-  SV_HIP(launch_zero(G->counts.p, G->n_bins, st));
+                          grad_axis, G->counts.p, G->n_bins, st));
   SV_HIP(launch_pose_bin(n, centers_cart, axes, G->Hinv, G->m, G->bin.p, G->rank.p, G->counts.p,
                          B->d_bad.p, st));
   SV_HIP(launch_pose_tile_scan(G->counts.p, G->n_bins, 0, G->first.p, B->d_chunks.p,
@@ -1063,6 +1066,31 @@ int pose_grid_enqueue(PoseGrid* G, hipStream_t st, int n, const double* centers_
   SV_HIP(launch_pose_scatter(n, centers_cart, axes, G->H, G->Hinv, G->m, G->bin.p, G->rank.p,
                              G->first.p, S->d_center_rel.p, S->d_center_id.p, S->d_tile_of.p,
                              S->d_pose_axes64.p, S->d_pose_axes32.p, st));
+  const bool grad = grad_center || grad_axis;
+  SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)max_chunks, G->n_chunks.p,
+                     B->d_offsets.p, B->d_list32.p, B->d_list64.p, st));
+  G->binned_n = n;
+  return 0;
+}
+
+// Evaluates the poses binned by the grid's last enqueue again (new outputs,
+// precision or gradient choice), without binning them again.  The binned
+// copies live in the grid, so the caller's pose arrays are not read.
+int pose_grid_reenqueue(PoseGrid* G, hipStream_t st, int precision, const double* grad_out,
+                        double* energy, double* grad_center, double* grad_axis) {
+  if (G->binned_n < 1) return fail(1, "no poses binned on this grid yet");
+  if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
+  Batch* B = G->B;
+  Poses* S = G->S;
+  SV_HIP(hipSetDevice(G->P->device));
+  if (G->last_stream_set && G->last_stream != st) SV_HIP(hipStreamSynchronize(G->last_stream));
+  G->last_stream = st;
+  G->last_stream_set = true;
+  const int n = G->binned_n;
+  const long long max_chunks = (long long)G->n_bins + (n + 63) / 64;
+  SV_HIP(launch_job_patch(B->d_jobs.p, S->d_center_rel.p, S->d_center_id.p, S->d_tile_of.p,
+                          S->d_pose_axes64.p, S->d_pose_axes32.p, n, grad_out, energy, grad_center,
+                          grad_axis, nullptr, 0, st));
   const bool grad = grad_center || grad_axis;
   SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)max_chunks, G->n_chunks.p,
                      B->d_offsets.p, B->d_list32.p, B->d_list64.p, st));
@@ -1109,6 +1137,13 @@ int sv_pose_grid_eval_async(void* grid, void* stream, int n, const double* cente
   return pose_grid_enqueue(static_cast<PoseGrid*>(grid), static_cast<hipStream_t>(stream), n,
                            centers_cart, axes, precision, grad_out, energy, grad_center,
                            grad_axis);
+}
+
+int sv_pose_grid_reeval_async(void* grid, void* stream, int precision, const double* grad_out,
+                              double* energy, double* grad_center, double* grad_axis) {
+  if (!grid) return fail(1, "null pose grid");
+  return pose_grid_reenqueue(static_cast<PoseGrid*>(grid), static_cast<hipStream_t>(stream),
+                             precision, grad_out, energy, grad_center, grad_axis);
 }
 
 int sv_pose_grid_check(void* grid) {

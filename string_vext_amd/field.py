@@ -295,22 +295,37 @@ class PoseGrid:
         _lib.check(self.lib.sv_pose_grid_create(plan.handle, int(n_expected), int(tile_centers),
                                                 ctypes.byref(handle)))
         self.handle = handle.value
+        # identifies the poses binned by the last eval (``token`` argument)
+        self.binned_token = None
 
     def eval(self, n, centers, axes, precision="fp64", grad_out=None, energy=None,
-             grad_center=None, grad_axis=None):
+             grad_center=None, grad_axis=None, token=None):
         """All arguments after ``n`` are device pointers (ints) or None."""
+        self.binned_token = None
         _lib.check(self.lib.sv_pose_grid_eval(self.handle, int(n), centers, axes,
                                               precision_code(precision), grad_out, energy,
                                               grad_center, grad_axis))
+        self.binned_token = token
 
     def eval_async(self, stream, n, centers, axes, precision="fp64", grad_out=None, energy=None,
-                   grad_center=None, grad_axis=None):
+                   grad_center=None, grad_axis=None, token=None):
         """Enqueue on ``stream`` (a raw hipStream_t handle, e.g. torch's
         ``current_stream().cuda_stream``; 0 = default stream) without waiting.
-        Invalid input and nonfinite results are reported by ``check``."""
+        Invalid input and nonfinite results are reported by ``check``.
+        ``token`` (any object) is kept as ``binned_token`` to name these poses."""
+        self.binned_token = None
         _lib.check(self.lib.sv_pose_grid_eval_async(self.handle, stream or None, int(n), centers,
                                                     axes, precision_code(precision), grad_out,
                                                     energy, grad_center, grad_axis))
+        self.binned_token = token
+
+    def reeval_async(self, stream, precision="fp64", grad_out=None, energy=None,
+                     grad_center=None, grad_axis=None):
+        """Evaluate the poses of the last eval again (new outputs) from the
+        grid's binned copies, without binning them again."""
+        _lib.check(self.lib.sv_pose_grid_reeval_async(self.handle, stream or None,
+                                                      precision_code(precision), grad_out, energy,
+                                                      grad_center, grad_axis))
 
     def check(self):
         """Wait for the last enqueued evaluation; raise on flagged input/results."""

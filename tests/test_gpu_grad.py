@@ -271,3 +271,51 @@ def test_pose_grid_async_on_torch_stream_and_flag_check(oracle, case, poses):
     assert torch.isnan(e[5]) or not torch.isfinite(e[5])
     with pytest.raises(RuntimeError, match="code 1"):
         check_poses(plan)
+
+
+def test_pose_grid_reeval_reuses_binning(case, poses):
+    """sv_pose_grid_reeval_async evaluates the last binned poses again without
+    reading the caller's arrays: bit-identical to a call that bins them anew.
+    autograd's backward takes that route only while the grid still holds its
+    forward's poses; after another forward on the grid it bins again."""
+    import torch
+    from string_vext_amd import _lib
+    from string_vext_amd.autograd import _grid_for, pose_energy
+    from string_vext_amd.field import PoseGrid
+    centers, axes = poses
+    n = len(centers)
+    plan = _plan(case)
+    grid = PoseGrid(plan, n_expected=n)
+    stream = torch.cuda.current_stream().cuda_stream
+    with pytest.raises(_lib.VextError, match="code 1"):
+        grid.reeval_async(stream, "fp64", energy=0)
+    c = torch.tensor(centers, dtype=torch.float64, device="cuda")
+    u = torch.tensor(axes, dtype=torch.float64, device="cuda")
+    for precision in ("fp64", "mixed"):
+        e = torch.empty(n, dtype=torch.float64, device="cuda")
+        ref = [torch.empty((n, 3), dtype=torch.float64, device="cuda") for _ in range(2)]
+        got = [torch.empty((n, 3), dtype=torch.float64, device="cuda") for _ in range(2)]
+        g = torch.linspace(-1.0, 2.0, n, dtype=torch.float64, device="cuda")
+        grid.eval_async(stream, n, c.data_ptr(), u.data_ptr(), precision, grad_out=g.data_ptr(),
+                        grad_center=ref[0].data_ptr(), grad_axis=ref[1].data_ptr())
+        grid.eval_async(stream, n, c.data_ptr(), u.data_ptr(), precision, energy=e.data_ptr())
+        c_saved = c.clone()
+        c.fill_(float("nan"))                # the binned copies are used, not c
+        grid.reeval_async(stream, precision, grad_out=g.data_ptr(),
+                          grad_center=got[0].data_ptr(), grad_axis=got[1].data_ptr())
+        grid.check()
+        c.copy_(c_saved)
+        assert torch.equal(got[0], ref[0]) and torch.equal(got[1], ref[1])
+    # autograd: forward A, forward B on the same grid, then backward of A
+    cg = torch.tensor(centers, dtype=torch.float64, device="cuda", requires_grad=True)
+    ug = torch.tensor(axes, dtype=torch.float64, device="cuda", requires_grad=True)
+    ea = pose_energy(plan, cg, ug, "fp64")
+    eb = pose_energy(plan, cg.detach().flip(0), ug.detach().flip(0), "fp64")
+    assert _grid_for(plan, n).binned_token is not None
+    ea.sum().backward()
+    gc_a, gu_a = cg.grad.clone(), ug.grad.clone()
+    cg.grad = None
+    ug.grad = None
+    pose_energy(plan, cg, ug, "fp64").sum().backward()      # reuses the binning
+    assert torch.equal(cg.grad, gc_a) and torch.equal(ug.grad, gu_a)
+    assert torch.equal(eb.flip(0), ea.detach())
