@@ -716,6 +716,9 @@ constexpr int kPoseWaves = 4;
 constexpr int kPoseAccN = 15;   // doubles in a PoseAcc (E, G0, G1, Rc, Ru, Gc, Gu)
 
 // FP64 pair energy and gradient contribution with explicit d = o u - r.
+// FAST (the near-contact terms of the mixed form) takes 1/d^2 from v_rcp_f64
+// and one Newton step instead of the IEEE division of the reference form.
+template <bool FAST = false>
 __device__ __forceinline__ void pose_pair64(double o, double ux, double uy, double uz, double rx,
                                             double ry, double rz, double rc2, double c12,
                                             double c6, double sh, double fl2, bool grad,
@@ -725,7 +728,7 @@ __device__ __forceinline__ void pose_pair64(double o, double ux, double uy, doub
   if (d2 < rc2) {
     const bool floored = d2 < fl2;
     const double safe = floored ? fl2 : d2;
-    const double inv = 1.0 / safe;
+    const double inv = FAST ? inv_fast(safe) : 1.0 / safe;
     const double i6 = inv * inv * inv;
     a.E += c12 * i6 * i6 - c6 * i6 + sh;
     if (grad && !floored) {
@@ -777,12 +780,11 @@ __global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __rest
     for (long long base = beg + 64 * wv; base < end; base += 64 * kPoseWaves) {
       const long long idx = base + lane;
       if (idx < end) {
+        // the FP64 copy serves the FP64 form and the mixed form's near terms
         if (MIXED) s_e[lane] = list32[idx];
-        else {
-          s_d[0][lane] = list64[3 * idx];
-          s_d[1][lane] = list64[3 * idx + 1];
-          s_d[2][lane] = list64[3 * idx + 2];
-        }
+        s_d[0][lane] = list64[3 * idx];
+        s_d[1][lane] = list64[3 * idx + 1];
+        s_d[2][lane] = list64[3 * idx + 2];
       }
       wave_sync();
       const int n = (int)(end - base < 64 ? end - base : 64);
@@ -822,11 +824,9 @@ __global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __rest
               aRu0 = fmaf(v, rx, aRu0); aRu1 = fmaf(v, ry, aRu1); aRu2 = fmaf(v, rz, aRu2);
             }
             if (near) {
-              const long long id = base + q;
-              const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
-                           qz = list64[3 * id + 2] - cz;
-              pose_pair64(o0, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
-              pose_pair64(o1, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
+              const double qx = s_d[0][q] - cx, qy = s_d[1][q] - cy, qz = s_d[2][q] - cz;
+              pose_pair64<true>(o0, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
+              pose_pair64<true>(o1, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
             }
           }
           a.E = fma(c6, (double)aE, a.E);
