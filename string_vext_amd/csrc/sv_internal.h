@@ -39,6 +39,9 @@ struct Job {
                         // cutoff for every orientation
   float inv_rc2f;       // 1/rc^2: shell atoms of the mixed path work in units of rc^2
   float m2o0sf, m2o1sf; // -2*o0/rc^2, -2*o1/rc^2
+  // the pose form works in units of 1/s, s = inv_rc2f exactly:
+  float o0sxf, o1sxf;   // o^2 s
+  float m2o0xf, m2o1xf; // -2 o s
   int sym;              // o1 == -o0 exactly
   // ---- per-type coefficients (index = framework LJ type) ------------------
   const double* c12;    // 4 eps sigma^12          (mixed Lorentz-Berthelot)
@@ -55,6 +58,8 @@ struct Job {
                         // orientation reaches the contact floor
   const double* c6s;    // c6 / rc^6            (shell form of the mixed path)
   const float* s6sf;    // sigma^6 / rc^6 in FP32
+  const double* c6x;    // c6 s^3               (pose form, s = inv_rc2f)
+  const float* s6xf;    // sigma^6 s^3 in FP32
   // ---- pose set -----------------------------------------------------------
   const double* center_rel;  // [n_centers][3] Cartesian offset from tile centre (sorted)
   const int* center_id;      // sorted slot -> caller's center index

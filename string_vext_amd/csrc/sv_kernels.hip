@@ -764,19 +764,33 @@ __global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __rest
                cz = J.center_rel[3 * slot + 2];
   const double ux = J.pose_axes64[3 * slot], uy = J.pose_axes64[3 * slot + 1],
                uz = J.pose_axes64[3 * slot + 2];
-  const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
-  const float uxf = (float)ux, uyf = (float)uy, uzf = (float)uz;
+  float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
+  float uxf = (float)ux, uyf = (float)uy, uzf = (float)uz;
+  // keep the FP32 copies in registers: left to itself the compiler converts
+  // them again on every candidate
+  asm volatile("" : "+v"(cxf), "+v"(cyf), "+v"(czf), "+v"(uxf), "+v"(uyf), "+v"(uzf));
   const double o0 = J.o0, o1 = J.o1, rc2 = J.rc2;
-  const float rc2f = J.rc2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
-  const float o0sq = J.o0sqf, o1sq = J.o1sqf;
+  // FP32 pairs work in units of 1/s (s = FP32 1/rc^2): x = s d^2 is clamped to
+  // [0, 1] by the clamp bit of the fma that forms it, so a pair beyond the
+  // cutoff contributes the clamped term, which the +shift added for every
+  // FP32 candidate cancels (up to ~3e-8 K per pair from the rounding of s).
+  const float sc = J.inv_rc2f, m2o0 = J.m2o0xf, m2o1 = J.m2o1xf;
+  float o0s = J.o0sxf, o1s = J.o1sxf;
+  // held in VGPRs: an fma reads at most one scalar register
+  asm volatile("" : "+v"(o0s), "+v"(o1s));
   const float o0f = (float)o0, o1f = (float)o1;
   const long long seg0 = J.seg_base + (long long)ch.tile * J.n_types;
   PoseAcc a = {};
   for (int t = 0; t < J.n_types; ++t) {
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
     const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
-    const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
-    int n_in = 0;   // in-cutoff FP32 pairs of this type
+    // c6/d^6 (sigma^6/d^6 - 1) = c6 s^3 i^3 (sigma^6 s^3 i^3 - 1), i = 1/x;
+    // dphi/d(d^2) = 3 c6 s^4 i^4 (1 - 2 sigma^6 s^3 i^3)
+    const double c6x = J.c6x[t], k3 = 3.0 * c6x * (double)sc;
+    const float s6x = J.s6xf[t], m2s6x = -2.0f * J.s6xf[t];
+    // near-contact candidates, and all within reach of the contact floor, are FP64
+    const float near2 = fmaxf(J.near2f[t], J.lo2f[t]);
+    long long n_fp = 0;   // FP32 candidates of this type (each gets +shift per site)
     for (long long base = beg + 64 * wv; base < end; base += 64 * kPoseWaves) {
       const long long idx = base + lane;
       if (idx < end) {
@@ -789,9 +803,8 @@ __global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __rest
       wave_sync();
       const int n = (int)(end - base < 64 ? end - base : 64);
       if (MIXED) {
-        // FP32 partials over at most 16 atoms, then FP64.  Only in-cutoff FP32
-        // pairs contribute (and are counted for the shift), so candidates far
-        // outside the cutoff add nothing, not a rounded -shift.
+        // FP32 partials over at most 16 atoms, then FP64.
+        n_fp += n;
         for (int q0 = 0; q0 < n; q0 += 16) {
           const int q1 = q0 + 16 < n ? q0 + 16 : n;
           float aE = 0.f, aG0 = 0.f, aG1 = 0.f, aRc0 = 0.f, aRc1 = 0.f, aRc2 = 0.f, aRu0 = 0.f,
@@ -799,24 +812,22 @@ __global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __rest
           for (int q = q0; q < q1; ++q) {
             const float4 e = s_e[q];
             const float rx = e.x - cxf, ry = e.y - cyf, rz = e.z - czf;
-            const float rr = rx * rx + ry * ry + rz * rz;
+            const float rr0 = rx * rx + ry * ry + rz * rz;
             const float tt = rx * uxf + ry * uyf + rz * uzf;
-            const bool near = rr < near2;
-            const float d0r = fmaf(tt, m2o0, rr + o0sq);
-            const float d1r = fmaf(tt, m2o1, rr + o1sq);
-            const bool in0 = !near && d0r < rc2f;
-            const bool in1 = !near && d1r < rc2f;
-            const float d0 = __builtin_amdgcn_fmed3f(d0r, fl2f, rc2f);
-            const float d1 = __builtin_amdgcn_fmed3f(d1r, fl2f, rc2f);
-            const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1);
+            const bool near = rr0 < near2;
+            // a near candidate's FP32 terms clamp like pairs beyond the cutoff
+            // (and are cancelled the same way); its FP64 terms follow below
+            const float rr = near ? 3.0e38f : rr0;
+            const float x0 = __builtin_amdgcn_fmed3f(fmaf(tt, m2o0, fmaf(rr, sc, o0s)), 0.0f, 1.0f);
+            const float x1 = __builtin_amdgcn_fmed3f(fmaf(tt, m2o1, fmaf(rr, sc, o1s)), 0.0f, 1.0f);
+            const float i0 = __builtin_amdgcn_rcpf(x0), i1 = __builtin_amdgcn_rcpf(x1);
             const float i03 = i0 * i0 * i0, i13 = i1 * i1 * i1;
-            aE += in0 ? i03 * fmaf(i03, s6, -1.0f) : 0.0f;
-            aE += in1 ? i13 * fmaf(i13, s6, -1.0f) : 0.0f;
-            n_in += (int)in0 + (int)in1;
+            aE = fmaf(i03, fmaf(i03, s6x, -1.0f), aE);
+            aE = fmaf(i13, fmaf(i13, s6x, -1.0f), aE);
             if (GRAD) {
-              const bool f0 = in0 && d0r > fl2f, f1 = in1 && d1r > fl2f;
-              const float g0 = f0 ? i03 * i0 * fmaf(-2.0f * s6, i03, 1.0f) : 0.0f;
-              const float g1 = f1 ? i13 * i1 * fmaf(-2.0f * s6, i13, 1.0f) : 0.0f;
+              // no gradient beyond the cutoff (x clamped to 1)
+              const float g0 = x0 < 1.0f ? i03 * i0 * fmaf(m2s6x, i03, 1.0f) : 0.0f;
+              const float g1 = x1 < 1.0f ? i13 * i1 * fmaf(m2s6x, i13, 1.0f) : 0.0f;
               aG0 += g0;
               aG1 += g1;
               const float w = g0 + g1, v = fmaf(o0f, g0, o1f * g1);
@@ -829,9 +840,9 @@ __global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __rest
               pose_pair64<true>(o1, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
             }
           }
-          a.E = fma(c6, (double)aE, a.E);
+          a.E = fma(c6x, (double)aE, a.E);
           if (GRAD) {
-            const double k = 3.0 * c6;
+            const double k = k3;
             a.G0 = fma(k, (double)aG0, a.G0);
             a.G1 = fma(k, (double)aG1, a.G1);
             a.Rc[0] = fma(k, (double)aRc0, a.Rc[0]);
@@ -851,7 +862,7 @@ __global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __rest
       }
       wave_sync();
     }
-    if (MIXED) a.E = fma((double)n_in, sh, a.E);
+    if (MIXED) a.E = fma((double)(2 * n_fp), sh, a.E);
   }
   // Waves 1.. hand their sums to wave 0, which adds them in wave order.
   if (wv > 0) {

@@ -97,6 +97,12 @@ void plan_fill_job(const Plan& P, Job& J) {
   J.lo2f = P.d_lo2f.p;
   J.c6s = P.d_c6s.p;
   J.s6sf = P.d_s6sf.p;
+  J.c6x = P.d_c6x.p;
+  J.s6xf = P.d_s6xf.p;
+  J.o0sxf = (float)(P.o0 * P.o0 * (double)(float)(1.0 / J.rc2));
+  J.o1sxf = (float)(P.o1 * P.o1 * (double)(float)(1.0 / J.rc2));
+  J.m2o0xf = (float)(-2.0 * P.o0 * (double)(float)(1.0 / J.rc2));
+  J.m2o1xf = (float)(-2.0 * P.o1 * (double)(float)(1.0 / J.rc2));
   J.inv_rc2f = (float)(1.0 / J.rc2);
   J.m2o0sf = (float)(-2.0 * P.o0 / J.rc2);
   J.m2o1sf = (float)(-2.0 * P.o1 / J.rc2);
@@ -554,6 +560,11 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
     const double rc6 = cutoff * cutoff * cutoff * cutoff * cutoff * cutoff;
     P->c6s.push_back(c6 / rc6);
     P->s6sf.push_back((float)(c12 / c6 / rc6));
+    // ... and of the pose form, in units of 1/s with s the FP32 value of 1/rc^2
+    // itself, so that only the clamp point (not the terms) carries its rounding
+    const double s = (double)(float)(1.0 / (cutoff * cutoff));
+    P->c6x.push_back(c6 * s * s * s);
+    P->s6xf.push_back((float)(c12 / c6 * s * s * s));
     float near2 = 0.0f;
     if (near_energy > 0.0) {
       // Repulsive-branch distance where 4 eps (x^2 - x) = near_energy, x = (sigma/r)^6.
@@ -592,6 +603,8 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
       (e = P->d_lo2f.upload(P->lo2f.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = P->d_c6s.upload(P->c6s.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = P->d_s6sf.upload(P->s6sf.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_c6x.upload(P->c6x.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_s6xf.upload(P->s6xf.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
     delete P;
     return hip_fail(e, "plan upload");
