@@ -132,17 +132,41 @@ __device__ __forceinline__ bool image_inside(const double d[3], int na, int nb, 
 // ---------------------------------------------------------------------------
 // Tile candidate lists
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void tile_count_kernel(const Job* __restrict__ jobs,
-                                                        const Tile* __restrict__ tiles,
-                                                        const long long* __restrict__ seg_base,
-                                                        int* __restrict__ counts) {
-  const int g = blockIdx.x, lane = threadIdx.x;
+// One block of kListThreads threads per tile; thread i takes atoms i,
+// i + kListThreads, ... of each type.
+constexpr int kListThreads = 256;
+
+// Exclusive prefix over the block's threads in thread order; `total` gets the
+// block sum.  Ends with a barrier, so it can be called again at once.
+__device__ __forceinline__ int list_block_scan(int v, int* total) {
+  __shared__ int s_w[kListThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int total_w;
+  const int x = wave_exclusive_scan(v, lane, &total_w);
+  if (lane == 0) s_w[w] = total_w;
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kListThreads / 64; ++k) {
+    before += k < w ? s_w[k] : 0;
+    all += s_w[k];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x;
+}
+
+__global__ __launch_bounds__(kListThreads) void tile_count_kernel(const Job* __restrict__ jobs,
+                                                                  const Tile* __restrict__ tiles,
+                                                                  const long long* __restrict__ seg_base,
+                                                                  int* __restrict__ counts) {
+  const int g = blockIdx.x;
   const Tile& T = tiles[g];
   const Job& J = jobs[T.job];
   const double r2 = T.list_r * T.list_r;
   for (int t = 0; t < J.n_types; ++t) {
     int count = 0;
-    for (int a = J.type_start[t] + lane; a < J.type_start[t + 1]; a += 64) {
+    for (int a = J.type_start[t] + threadIdx.x; a < J.type_start[t + 1]; a += kListThreads) {
       double d[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) d[k] = J.frac[3 * a + k] - T.frac[k];
@@ -154,9 +178,9 @@ __global__ __launch_bounds__(64) void tile_count_kernel(const Job* __restrict__ 
             count += image_inside(d, na, nb, nc, J.H, r2, &x, &y, &z);
           }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) count += __shfl_xor(count, o, 64);
-    if (lane == 0) counts[seg_base[g] + t] = count;
+    int total;
+    (void)list_block_scan(count, &total);
+    if (threadIdx.x == 0) counts[seg_base[g] + t] = total;
   }
 }
 
@@ -240,8 +264,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const int* __r
   int v[kScanItems];
   const long long first = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
   long long total;
-  long long run = sums[blockIdx.x] +
+  long long run = (sums ? sums[blockIdx.x] : 0) +
                   block_exclusive_scan(scan_thread_items(counts, n, first, v), &total);
+  // a single chunk (sums == nullptr) is scanned by this one launch
+  if (!sums && threadIdx.x == 0) offsets[n] = total;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     if (first + j < n) offsets[first + j] = run;
@@ -249,13 +275,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const int* __r
   }
 }
 
-__global__ __launch_bounds__(64) void tile_fill_kernel(const Job* __restrict__ jobs,
-                                                       const Tile* __restrict__ tiles,
-                                                       const long long* __restrict__ seg_base,
-                                                       const long long* __restrict__ offsets,
-                                                       float4* __restrict__ list32,
-                                                       double* __restrict__ list64) {
-  const int g = blockIdx.x, lane = threadIdx.x;
+__global__ __launch_bounds__(kListThreads) void tile_fill_kernel(const Job* __restrict__ jobs,
+                                                                 const Tile* __restrict__ tiles,
+                                                                 const long long* __restrict__ seg_base,
+                                                                 const long long* __restrict__ offsets,
+                                                                 float4* __restrict__ list32,
+                                                                 double* __restrict__ list64) {
+  const int g = blockIdx.x;
   const Tile& T = tiles[g];
   const Job& J = jobs[T.job];
   const double r2 = T.list_r * T.list_r;
@@ -263,8 +289,8 @@ __global__ __launch_bounds__(64) void tile_fill_kernel(const Job* __restrict__ j
     long long base = offsets[seg_base[g] + t];
     const long long seg_end = offsets[seg_base[g] + t + 1];
     const int a_end = J.type_start[t + 1];
-    for (int a0 = J.type_start[t]; a0 < a_end; a0 += 64) {
-      const int a = a0 + lane;
+    for (int a0 = J.type_start[t]; a0 < a_end; a0 += kListThreads) {
+      const int a = a0 + (int)threadIdx.x;
       const bool ok = a < a_end;
       double d[3] = {0.0, 0.0, 0.0};
       ShiftBox b = {{0, 0, 0}, {-1, -1, -1}};
@@ -273,7 +299,7 @@ __global__ __launch_bounds__(64) void tile_fill_kernel(const Job* __restrict__ j
         for (int k = 0; k < 3; ++k) d[k] = J.frac[3 * a + k] - T.frac[k];
         b = shift_box(d, J.hn, T.list_r);
       }
-      // Pass 1: how many images this lane emits.
+      // Pass 1: how many images this thread emits.
       int mine = 0;
       for (int na = b.lo[0]; na <= b.hi[0]; ++na)
         for (int nb = b.lo[1]; nb <= b.hi[1]; ++nb)
@@ -282,8 +308,8 @@ __global__ __launch_bounds__(64) void tile_fill_kernel(const Job* __restrict__ j
             mine += image_inside(d, na, nb, nc, J.H, r2, &x, &y, &z);
           }
       int total;
-      long long slot = base + wave_exclusive_scan(mine, lane, &total);
-      // Pass 2: write them in lane order (deterministic list order), never
+      long long slot = base + list_block_scan(mine, &total);
+      // Pass 2: write them in atom order (deterministic list order), never
       // past the end of this (tile, type) segment.
       for (int na = b.lo[0]; na <= b.hi[0]; ++na)
         for (int nb = b.lo[1]; nb <= b.hi[1]; ++nb)
@@ -1236,7 +1262,7 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
 hipError_t launch_tile_count(const Job* d_jobs, const Tile* d_tiles, int n_tiles_total,
                              const long long* d_seg_base, int* d_counts, hipStream_t s) {
   if (n_tiles_total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(tile_count_kernel, dim3(n_tiles_total), dim3(64), 0, s, d_jobs, d_tiles,
+  hipLaunchKernelGGL(tile_count_kernel, dim3(n_tiles_total), dim3(kListThreads), 0, s, d_jobs, d_tiles,
                      d_seg_base, d_counts);
   return hipGetLastError();
 }
@@ -1246,12 +1272,17 @@ long long scan_scratch_size(long long n) { return (n + kScanChunk - 1) / kScanCh
 hipError_t launch_scan(const int* d_counts, long long n, long long* d_offsets, long long* d_sums,
                        hipStream_t s) {
   const long long nb = (n + kScanChunk - 1) / kScanChunk;
-  if (nb > 0) hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s,
-                                 d_counts, n, d_sums);
+  if (nb <= 1) {
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(1), dim3(kScanThreads), 0, s, d_counts, n,
+                       (const long long*)nullptr, d_offsets);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, d_counts, n,
+                     d_sums);
   hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads), 0, s, d_sums, nb, n,
                      d_offsets);
-  if (nb > 0) hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s,
-                                 d_counts, n, d_sums, d_offsets);
+  hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, d_counts, n,
+                     d_sums, d_offsets);
   return hipGetLastError();
 }
 
@@ -1259,7 +1290,7 @@ hipError_t launch_tile_fill(const Job* d_jobs, const Tile* d_tiles, int n_tiles_
                             const long long* d_seg_base, const long long* d_offsets,
                             float4* d_list32, double* d_list64, hipStream_t s) {
   if (n_tiles_total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(tile_fill_kernel, dim3(n_tiles_total), dim3(64), 0, s, d_jobs, d_tiles,
+  hipLaunchKernelGGL(tile_fill_kernel, dim3(n_tiles_total), dim3(kListThreads), 0, s, d_jobs, d_tiles,
                      d_seg_base, d_offsets, d_list32, d_list64);
   return hipGetLastError();
 }
