@@ -81,6 +81,7 @@ struct Job {
   float* grid32;             // [n_centers] asinh(marginal/1000)
   float* site32;             // [n_centers] single-site field
   double temperature;
+  double log_n_axes;         // ln(n_axes), for the marginal
   unsigned long long* pair_count;  // census output (pairs with d^2 < rc^2)
   int* bad;                  // set to 1 when a nonfinite energy is produced
   int pad2;

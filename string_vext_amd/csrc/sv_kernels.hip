@@ -83,6 +83,11 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -349,7 +354,7 @@ __device__ __forceinline__ void setup_item(const Job* __restrict__ jobs, int n_j
   c.J = jobs + j;
   const long long local = w - c.J->work_begin;
   const int ng = c.J->n_groups;
-  c.slot = uniform((int)(local / ng));
+  c.slot = uniform(ng == 1 ? (int)local : (int)(local / ng));
   c.group = uniform((int)(local - (long long)c.slot * ng));
   c.tile = uniform(c.J->tile_of[c.slot]);
   const int lane = threadIdx.x & 63;
@@ -393,7 +398,11 @@ __device__ __forceinline__ long long next_item(unsigned long long* __restrict__ 
 }
 
 // Pose energies, fused marginal/grid, and the site field at the center
-// (lane partials `site`, already including every shift term).
+// (lane partials `site`, already including every shift term).  FAST (the mixed
+// form) takes the Boltzmann terms exp(a - max) <= 1, their sum and its log in
+// FP32: ~1e-7 relative on a sum of at most 64 terms, below the FP32 output's
+// rounding.
+template <bool FAST = false>
 __device__ __forceinline__ void write_pose(const WaveCtx& c, double E, bool with_site, double site) {
   const Job& J = *c.J;
   const int cid = J.center_id[c.slot];
@@ -407,9 +416,15 @@ __device__ __forceinline__ void write_pose(const WaveCtx& c, double E, bool with
     const double T = J.temperature;
     const double a = c.k_ok ? -E / T : -INFINITY;
     const double m = wave_max(a);
-    const double s = wave_sum(c.k_ok ? exp(a - m) : 0.0);
+    double ls;
+    if (FAST) {
+      const float s = wave_sum(c.k_ok ? expf((float)(a - m)) : 0.0f);
+      ls = (double)logf(s);
+    } else {
+      ls = log(wave_sum(c.k_ok ? exp(a - m) : 0.0));
+    }
     if ((threadIdx.x & 63) == 0) {
-      const double marg = -T * (m + log(s) - log((double)J.n_axes));
+      const double marg = -T * (m + ls - J.log_n_axes);
       const float mf = (float)marg;
       J.marg32[cid] = mf;
       if (J.grid32) J.grid32[cid] = asinhf(mf / 1000.0f);
@@ -638,7 +653,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     S_shift = fma((double)kept_far, sh, S_shift);
     S = fma(c6, (double)Sf, S);
   }
-  write_pose(c, E, do_site, lane == 0 ? S + S_shift : S);
+  write_pose<true>(c, E, do_site, lane == 0 ? S + S_shift : S);
 }
 
 template <bool SYM>
@@ -1126,6 +1141,8 @@ __global__ __launch_bounds__(64) void marginal_kernel(const Job* __restrict__ jo
   if (g >= n_total) return;
   const int j = find_job(job_first, n_jobs, g);
   const Job& J = jobs[j];
+  // jobs of at most 64 axes got their marginal from the field kernel
+  if (J.n_groups == 1 || !J.marg32) return;
   const int cid = (int)(g - job_first[j]);
   const int lane = threadIdx.x;
   const double T = J.temperature;
@@ -1136,7 +1153,7 @@ __global__ __launch_bounds__(64) void marginal_kernel(const Job* __restrict__ jo
   for (int k = lane; k < J.n_axes; k += 64) s += exp(-J.out64[(long long)cid * J.n_axes + k] / T - m);
   s = wave_sum(s);
   if (lane == 0) {
-    const double marg = -T * (m + log(s) - log((double)J.n_axes));
+    const double marg = -T * (m + log(s) - J.log_n_axes);
     const float mf = (float)marg;
     J.marg32[cid] = mf;
     if (J.grid32) J.grid32[cid] = asinhf(mf / 1000.0f);

@@ -272,6 +272,7 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
     J.grid32 = grid ? grid[j] : nullptr;
     J.site32 = site ? site[j] : nullptr;
     J.temperature = temps ? temps[j] : 0.0;
+    J.log_n_axes = std::log((double)J.n_axes);
     if ((J.marg32 || J.grid32) && !(temps && std::isfinite(temps[j]) && temps[j] > 0.0)) {
       delete B;
       return fail(1, "positive finite temperature required for the marginal");
