@@ -14,6 +14,17 @@
 
 namespace sv {
 
+// 1/x in FP64 from v_rcp_f64 and one Newton step.  The step squares the
+// seed's relative error (1 - x y1 = (1 - x y0)^2), so even a seed good to only
+// 2^-20 gives ~1e-12.  For terms that need FP64-level relative accuracy but
+// not IEEE rounding: the mixed kernels' near-contact terms and the String
+// images.
+__device__ __forceinline__ double inv_fast(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+
+
 // One field job = (framework, guest, pose set, axes, outputs).  A single
 // Vext field is a batch of one job; a ragged batch is many jobs launched
 // together.  Lives in device global memory and is read with uniform

@@ -457,15 +457,6 @@ __device__ __forceinline__ double pair64(double o, double ux, double uy, double 
   return e;
 }
 
-// 1/x in FP64 from v_rcp_f64 and one Newton step.  The step squares the
-// seed's relative error (1 - x y1 = (1 - x y0)^2), so even a seed good to only
-// 2^-20 gives ~1e-12; the mixed kernel's near-contact terms need FP64-level
-// relative accuracy (they are summed with FP32-path terms), not IEEE rounding.
-__device__ __forceinline__ double inv_fast(double x) {
-  const double y = __builtin_amdgcn_rcp(x);
-  return fma(y, fma(-x, y, 1.0), y);
-}
-
 // Same pair term from a precomputed squared distance (mixed near path).
 __device__ __forceinline__ double pair64_d2(double d2, double rc2, double c12, double c6,
                                             double sh, double fl2) {
