@@ -748,8 +748,9 @@ constexpr int kPoseWaves = 4;
 constexpr int kPoseAccN = 15;   // doubles in a PoseAcc (E, G0, G1, Rc, Ru, Gc, Gu)
 
 // FP64 pair energy and gradient contribution with explicit d = o u - r.
-// FAST (the near-contact terms of the mixed form) takes 1/d^2 from v_rcp_f64
-// and one Newton step instead of the IEEE division of the reference form.
+// FAST takes 1/d^2 from v_rcp_f64 and one Newton step instead of the IEEE
+// division of the reference form; the pose kernels use it in both forms (no
+// reference counterpart asks for IEEE rounding there).
 template <bool FAST = false>
 __device__ __forceinline__ void pose_pair64(double o, double ux, double uy, double uz, double rx,
                                             double ry, double rz, double rc2, double c12,
@@ -888,8 +889,8 @@ __global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __rest
       } else {
         for (int q = 0; q < n; ++q) {
           const double rx = s_d[0][q] - cx, ry = s_d[1][q] - cy, rz = s_d[2][q] - cz;
-          pose_pair64(o0, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2, GRAD, a);
-          pose_pair64(o1, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2, GRAD, a);
+          pose_pair64<true>(o0, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2, GRAD, a);
+          pose_pair64<true>(o1, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2, GRAD, a);
         }
       }
       wave_sync();
