@@ -54,6 +54,7 @@ struct Job {
   float o0sxf, o1sxf;   // o^2 s
   float m2o0xf, m2o1xf; // -2 o s
   int sym;              // o1 == -o0 exactly
+  int near_inside;      // both sites of every near-contact atom are inside the cutoff
   // ---- per-type coefficients (index = framework LJ type) ------------------
   const double* c12;    // 4 eps sigma^12          (mixed Lorentz-Berthelot)
   const double* c6;     // 4 eps sigma^6

@@ -467,6 +467,14 @@ __device__ __forceinline__ double pair64_d2(double d2, double rc2, double c12, d
   return d2 < rc2 ? e : 0.0;
 }
 
+// The same term for a pair known to be inside the cutoff.
+__device__ __forceinline__ double pair64_in(double d2, double c12, double c6, double sh,
+                                            double fl2) {
+  const double inv = inv_fast(d2 < fl2 ? fl2 : d2);
+  const double i6 = inv * inv * inv;
+  return fma(c12 * i6, i6, fma(-c6, i6, sh));
+}
+
 // Atoms per FP32 partial sum in the mixed kernel.
 constexpr int kFlush = 16;
 
@@ -533,12 +541,21 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         if (do_site) S += pair64_d2(RR, rc2, c12, c6, sh, fl2);
       }
       wave_sync();
-      for (int q = 0; q < n_near; ++q) {
-        // FP64 near-contact pair: one dot product serves both sites.
-        const double4 d = L.near[wv][q];
-        const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
-        E += pair64_d2(fma(tt, m2o0d, d.w + o0sqd), rc2, c12, c6, sh, fl2);
-        E += pair64_d2(fma(tt, m2o1d, d.w + o1sqd), rc2, c12, c6, sh, fl2);
+      // FP64 near-contact pairs: one dot product serves both sites.
+      if (J.near_inside) {
+        for (int q = 0; q < n_near; ++q) {
+          const double4 d = L.near[wv][q];
+          const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
+          E += pair64_in(fma(tt, m2o0d, d.w + o0sqd), c12, c6, sh, fl2);
+          E += pair64_in(fma(tt, m2o1d, d.w + o1sqd), c12, c6, sh, fl2);
+        }
+      } else {
+        for (int q = 0; q < n_near; ++q) {
+          const double4 d = L.near[wv][q];
+          const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
+          E += pair64_d2(fma(tt, m2o0d, d.w + o0sqd), rc2, c12, c6, sh, fl2);
+          E += pair64_d2(fma(tt, m2o1d, d.w + o1sqd), rc2, c12, c6, sh, fl2);
+        }
       }
       wave_sync();
       n_near = 0;

@@ -49,6 +49,7 @@ struct Plan {
   std::vector<double> c12, c6, shift, floor2, sigma, eps, c6s, c6x;
   std::vector<float> s6f, floor2f, near2f, lo2f, s6sf, s6xf;
   double o0, o1, cutoff, keep_r, near_energy;
+  bool near_inside = false;   // every FP64 near-contact pair is inside the cutoff
   DevBuf<double> d_frac, d_c12, d_c6, d_shift, d_floor2, d_sigma, d_eps, d_c6s, d_c6x;
   DevBuf<int> d_type_start;
   DevBuf<float> d_s6f, d_floor2f, d_near2f, d_lo2f, d_s6sf, d_s6xf;

@@ -85,6 +85,7 @@ void plan_fill_job(const Plan& P, Job& J) {
   const double in_r = P.cutoff - std::max(std::fabs(P.o0), std::fabs(P.o1)) - 1e-3;
   J.in2f = in_r > 0.0 ? (float)(in_r * in_r) : 0.0f;
   J.sym = (P.o1 == -P.o0);
+  J.near_inside = P.near_inside ? 1 : 0;
   J.c12 = P.d_c12.p;
   J.c6 = P.d_c6.p;
   J.shift = P.d_shift.p;
@@ -587,6 +588,13 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
     P->near2f.push_back(near2);
     const double lo = std::sqrt(fl) + omax + 1e-3;
     P->lo2f.push_back((float)(lo * lo));
+  }
+  // An atom is near-contact when its center distance is below
+  // sqrt(max(near2f, lo2f)); both of its sites are then within that + omax.
+  P->near_inside = true;
+  for (int t = 0; t < P->n_types; ++t) {
+    const double r = std::sqrt((double)std::max(P->near2f[t], P->lo2f[t]));
+    if (!(r * (1.0 + 1e-6) + omax < cutoff)) P->near_inside = false;
   }
   hipError_t e;
   if ((e = hipGetDevice(&P->device)) != hipSuccess || (e = hipStreamCreate(&P->stream)) != hipSuccess ||
