@@ -423,11 +423,54 @@ def workload_strings(args):
     print(json.dumps(line), flush=True)
 
 
+def workload_dropin(args):
+    """The reference's own call sequence with this library as its --kernel: the
+    full 60^3 x 64 field in 512-center batches, cKDTree CSR on the host, one
+    evaluate_reused_neighbors call per batch (FP64, host buffers, tables sent
+    with every call).  Compared with the batched FP64 field on the same poses."""
+    from string_vext_amd import _lib, synthetic
+    from string_vext_amd.field import Plan
+    from string_vext_amd.native import NativeReusePotential
+    from string_vext_amd.orientation import regular_pose_grid
+    case = synthetic.full_field_case()
+    centers, axes = regular_pose_grid(SIZE, ORIENTATIONS)
+    pot = NativeReusePotential(case, _lib.LIB_PATH, batched=False)
+    pot.evaluate(centers[:512], axes)                      # warm-up batch
+    walls, queries, computes, out = [], [], [], None
+    for _ in range(args.steps):
+        pot.query_seconds = pot.compute_seconds = 0.0
+        t = time.perf_counter()
+        out = pot.evaluate(centers, axes)
+        walls.append(time.perf_counter() - t)
+        queries.append(pot.query_seconds)
+        computes.append(pot.compute_seconds)
+    plan = Plan.from_case(case)
+    poses = plan.grid(SIZE, axes)
+    pairs = plan.census(poses)
+    ref = plan.field(poses, precision="fp64")["energy"]
+    rel = float((np.abs(out - ref) / np.maximum(np.abs(ref), 1.0)).max())
+    wall, query, compute = (float(np.mean(v)) for v in (walls, queries, computes))
+    line = {"metric": "pair-evals/s, 60^3 x 64 Vext through the reference's call sequence "
+                      "(evaluate_reused_neighbors per 512-center batch, host cKDTree CSR)",
+            "value": pairs / wall, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "full-size single Vext field: 60^3 grid x 64 orientations, "
+                                   "1200-atom triclinic framework, drop-in C ABI route",
+                       "batches": (len(centers) + 511) // 512},
+            "query_seconds": query, "call_seconds": compute,
+            "calls_value": pairs / compute,
+            "max_relative_vs_batched_fp64_floor1K": rel}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", choices=("field", "ragged", "poses", "strings"), default="field",
+    ap.add_argument("--workload", choices=("field", "ragged", "poses", "strings", "dropin"),
+                    default="field",
                     help="field (default): the 60^3 x 64 headline; the others measure the "
-                         "ragged batch, differentiable poses and batched strings")
+                         "ragged batch, differentiable poses, batched strings and the drop-in "
+                         "C ABI route")
     ap.add_argument("--iterations", type=int, default=2000, help="strings workload iterations")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -450,7 +493,8 @@ def main():
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
     if args.workload != "field":
-        {"ragged": workload_ragged, "poses": workload_poses, "strings": workload_strings}[args.workload](args)
+        {"ragged": workload_ragged, "poses": workload_poses, "strings": workload_strings,
+         "dropin": workload_dropin}[args.workload](args)
         return
 
     from string_vext_amd import synthetic
