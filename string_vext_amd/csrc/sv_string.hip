@@ -161,13 +161,9 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
       }
     }
   };
-  auto pair64 = [&](long long i, double c12, double c6, double sh, double fl2) {
-    const double q[3] = {IG.list64[3 * i], IG.list64[3 * i + 1], IG.list64[3 * i + 2]};
-    pair64r(q, c12, c6, sh, fl2);
-  };
   for (int t = 0; t < J.n_types; ++t) {
     const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
-    const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t];
+    const float near2 = J.near2f[t];
     const long long beg = IG.offsets[seg0 + t], end = IG.offsets[seg0 + t + 1];
     if (!MIXED) {
       // the next candidate's coordinates are loaded while this one is evaluated
@@ -185,13 +181,23 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
     }
     // Mixed: FP32 terms in the main sweep; near-contact candidates are queued
     // in LDS and evaluated in FP64 one per lane, so the sweep never branches
-    // into the FP64 path.
+    // into the FP64 path.  The FP32 terms work in units of 1/s (s = FP32
+    // 1/rc^2): x = s d^2 is clamped to [0, 1] by the fma clamp bit, so every
+    // candidate slot of the sweep -- beyond the cutoff, past the list's end or
+    // near-contact (forced) -- yields either its term or the clamped one,
+    // which the +shift added per slot cancels (~3e-8 K per clamped pair from
+    // the rounding of s).
     int nq = 0;
-    // Lane partials of this type in FP32: e in units of c6, gradient in K/A;
-    // flushed into the FP64 sums once per type.
+    // Lane partials of this type in FP32: e in units of c6 s^3, gradient in
+    // K/A; flushed into the FP64 sums once per type.
     float eacc = 0.f, gcf[3] = {0.f, 0.f, 0.f}, guf[3] = {0.f, 0.f, 0.f};
-    int kept = 0;
-    const float c6f6 = (float)(6.0 * c6);
+    int slots = 0;
+    const float sc = J.inv_rc2f, s6x = J.s6xf[t], m2s6x = -2.0f * s6x;
+    const float ox[2] = {J.o0sxf, J.o1sxf}, mx[2] = {J.m2o0xf, J.m2o1xf};
+    // 2 dphi/d(d^2) = 6 c6 s^4 i^4 (1 - 2 sigma^6 s^3 i^3), i = 1/x
+    const float c6f6 = (float)(6.0 * J.c6x[t] * (double)sc);
+    // candidates within the floor's reach are near-contact (FP64) too
+    const float near2x = fmaxf(near2, J.lo2f[t]);
     const float of0 = (float)o[0], of1 = (float)o[1];
     // the next chunk's entry is loaded while this chunk is evaluated
     float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -207,45 +213,44 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
         rf[2] = e.z - cf[2];
         rrf = rf[0] * rf[0] + rf[1] * rf[1] + rf[2] * rf[2];
       }
-      const bool keep = rrf < J.keep2f;
-      const bool near = keep && rrf < near2;
+      const bool near = rrf < near2x;   // false past the list's end (rrf = 3e38)
       const unsigned long long mnear = __ballot(near);
       if (near) s_near[wv][nq + lanes_below(mnear)] = (int)(i - beg);
       nq += __popcll(mnear);
-      if (keep && !near) {
-        // clamped to [floor2, rc^2]: a clamped pair gives -shift, compensated by
-        // the +shift added per candidate
-        ++kept;
-        const float tt = rf[0] * uf[0] + rf[1] * uf[1] + rf[2] * uf[2];
+      ++slots;
+      const float rr = near ? 3.0e38f : rrf;
+      const float tt = rf[0] * uf[0] + rf[1] * uf[1] + rf[2] * uf[2];
 #pragma unroll
-        for (int si = 0; si < 2; ++si) {
+      for (int si = 0; si < 2; ++si) {
+        const float x = __builtin_amdgcn_fmed3f(fmaf(tt, mx[si], fmaf(rr, sc, ox[si])), 0.0f, 1.0f);
+        const float iv = __builtin_amdgcn_rcpf(x);
+        const float i3 = iv * iv * iv;
+        eacc = fmaf(i3, fmaf(i3, s6x, -1.0f), eacc);
+        if (GRAD) {
+          // no gradient beyond the cutoff (x clamped to 1)
+          const float g = x < 1.0f ? c6f6 * (i3 * iv * fmaf(m2s6x, i3, 1.0f)) : 0.0f;
           const float of = si ? of1 : of0;
-          const float d2r = fmaf(tt, -2.0f * of, rrf + of * of);
-          const float d2 = __builtin_amdgcn_fmed3f(d2r, fl2f, J.rc2f);
-          const float iv = __builtin_amdgcn_rcpf(d2);
-          const float i3 = iv * iv * iv;
-          eacc = fmaf(i3, fmaf(i3, s6, -1.0f), eacc);
-          if (GRAD) {
-            const bool in = d2r > fl2f && d2r < J.rc2f;
-            const float g = in ? c6f6 * (i3 * iv * fmaf(-2.0f * s6, i3, 1.0f)) : 0.0f;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              const float dk = fmaf(of, uf[k], -rf[k]);
-              gcf[k] = fmaf(g, dk, gcf[k]);
-              guf[k] = fmaf(g * of, dk, guf[k]);
-            }
+          for (int k = 0; k < 3; ++k) {
+            const float dk = fmaf(of, uf[k], -rf[k]);
+            gcf[k] = fmaf(g, dk, gcf[k]);
+            guf[k] = fmaf(g * of, dk, guf[k]);
           }
         }
       }
       if (nq >= 64 || b0 + 64 >= end) {
         wave_barrier_s();
         for (int q0 = 0; q0 < nq; q0 += 64)
-          if (q0 + lane < nq) pair64(beg + s_near[wv][q0 + lane], c12, c6, sh, fl2);
+          if (q0 + lane < nq) {
+            const long long id = beg + s_near[wv][q0 + lane];
+            const double qv[3] = {IG.list64[3 * id], IG.list64[3 * id + 1], IG.list64[3 * id + 2]};
+            pair64r(qv, c12, c6, sh, fl2);
+          }
         wave_barrier_s();
         nq = 0;
       }
     }
-    E += fma(c6, (double)eacc, 2.0 * sh * kept);
+    E += fma(J.c6x[t], (double)eacc, 2.0 * sh * slots);
     if (GRAD)
       for (int k = 0; k < 3; ++k) {
         gc[k] += (double)gcf[k];
