@@ -188,3 +188,27 @@ def test_many_list_segments_multi_block_scan(small_case):
     b = plan.field(coarse, precision="fp64")["energy"]
     rel = np.abs(a - b) / np.maximum(np.abs(b), 1.0)
     assert rel.max() < 1e-12, rel.max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_short_cutoff_near_contact_pairs_reach_the_cutoff(oracle, small_case, precision):
+    """Cutoff 4.5 A: the near-contact radius (~4.4 A here) plus the site offset
+    reaches past the cutoff, so the mixed kernel's FP64 near-contact terms keep
+    their cutoff test (with the usual 12.8 A the plan drops it).  Energies
+    against the FP64 oracle."""
+    from string_vext_amd.field import Plan
+    case = dict(small_case)
+    case["cutoff"] = 4.5
+    centers, axes = oracle.regular_pose_grid(6, 64)
+    ref = oracle.reuse_energy(case, centers, axes)
+    plan = Plan.from_case(case)
+    out = plan.field(plan.grid(6, axes), precision=precision)["energy"]
+    m = oracle.error_metrics(ref, out)
+    if precision == "fp64":
+        assert m["all_max_relative_error_floor1K"] < 1e-12
+    else:
+        # (at 4.5 A every in-cutoff pair carries a shift of ~+40 K, so few or
+        # no poses fall under the accessible mask; the relative bound holds)
+        if m["accessible_poses"]:
+            assert m["accessible_max_abs_error_K"] < 1e-3
+        assert m["all_max_relative_error_floor1K"] < 1e-5
