@@ -449,7 +449,7 @@ __device__ __forceinline__ double pair64(double o, double ux, double uy, double 
   const double d2 = dx * dx + dy * dy + dz * dz;
   double e = 0.0;
   if (d2 < rc2) {
-    const double safe = d2 < fl2 ? fl2 : d2;
+    const double safe = fmax(d2, fl2);   // the floor: one v_max_f64
     const double inv = 1.0 / safe;
     const double i6 = inv * inv * inv;
     e = c12 * i6 * i6 - c6 * i6 + sh;
@@ -460,7 +460,7 @@ __device__ __forceinline__ double pair64(double o, double ux, double uy, double 
 // Same pair term from a precomputed squared distance (mixed near path).
 __device__ __forceinline__ double pair64_d2(double d2, double rc2, double c12, double c6,
                                             double sh, double fl2) {
-  const double safe = d2 < fl2 ? fl2 : d2;
+  const double safe = fmax(d2, fl2);   // the floor: one v_max_f64
   const double inv = inv_fast(safe);
   const double i6 = inv * inv * inv;
   const double e = fma(c12 * i6, i6, fma(-c6, i6, sh));
@@ -470,7 +470,7 @@ __device__ __forceinline__ double pair64_d2(double d2, double rc2, double c12, d
 // The same term for a pair known to be inside the cutoff.
 __device__ __forceinline__ double pair64_in(double d2, double c12, double c6, double sh,
                                             double fl2) {
-  const double inv = inv_fast(d2 < fl2 ? fl2 : d2);
+  const double inv = inv_fast(fmax(d2, fl2));
   const double i6 = inv * inv * inv;
   return fma(c12 * i6, i6, fma(-c6, i6, sh));
 }
@@ -777,7 +777,7 @@ __device__ __forceinline__ void pose_pair64(double o, double ux, double uy, doub
   const double d2 = dx * dx + dy * dy + dz * dz;
   if (d2 < rc2) {
     const bool floored = d2 < fl2;
-    const double safe = floored ? fl2 : d2;
+    const double safe = fmax(d2, fl2);
     const double inv = FAST ? inv_fast(safe) : 1.0 / safe;
     const double i6 = inv * inv * inv;
     a.E += c12 * i6 * i6 - c6 * i6 + sh;

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
       const double d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
       if (d2 < rc2) {
         const bool floored = d2 < fl2;
-        const double inv = inv_fast(floored ? fl2 : d2);
+        const double inv = inv_fast(fmax(d2, fl2));
         const double i6 = inv * inv * inv;
         E += c12 * i6 * i6 - c6 * i6 + sh;
         if (GRAD && !floored) {
