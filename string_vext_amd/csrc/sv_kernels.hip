@@ -512,7 +512,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const double rc2 = J.rc2;
   const float inv_rc2f = J.inv_rc2f, m2o0s = J.m2o0sf, m2o1s = J.m2o1sf;
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
-  const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1;
+  const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1, do1 = o1sqd - o0sqd;
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
 
   double E = 0.0, S = 0.0, S_shift = 0.0;
@@ -537,7 +537,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
                      qz = list64[3 * id + 2] - cz;
         const double RR = qx * qx + qy * qy + qz * qz;
-        L.near[wv][lane] = make_double4(qx, qy, qz, RR);
+        // w = |q|^2 + o0^2; the second site's base is w + (o1^2 - o0^2)
+        L.near[wv][lane] = make_double4(qx, qy, qz, RR + o0sqd);
         if (do_site) S += pair64_d2(RR, rc2, c12, c6, sh, fl2);
       }
       wave_sync();
@@ -546,15 +547,15 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         for (int q = 0; q < n_near; ++q) {
           const double4 d = L.near[wv][q];
           const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
-          E += pair64_in(fma(tt, m2o0d, d.w + o0sqd), c12, c6, sh, fl2);
-          E += pair64_in(fma(tt, m2o1d, d.w + o1sqd), c12, c6, sh, fl2);
+          E += pair64_in(fma(tt, m2o0d, d.w), c12, c6, sh, fl2);
+          E += pair64_in(fma(tt, m2o1d, SYM ? d.w : d.w + do1), c12, c6, sh, fl2);
         }
       } else {
         for (int q = 0; q < n_near; ++q) {
           const double4 d = L.near[wv][q];
           const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
-          E += pair64_d2(fma(tt, m2o0d, d.w + o0sqd), rc2, c12, c6, sh, fl2);
-          E += pair64_d2(fma(tt, m2o1d, d.w + o1sqd), rc2, c12, c6, sh, fl2);
+          E += pair64_d2(fma(tt, m2o0d, d.w), rc2, c12, c6, sh, fl2);
+          E += pair64_d2(fma(tt, m2o1d, SYM ? d.w : d.w + do1), rc2, c12, c6, sh, fl2);
         }
       }
       wave_sync();
