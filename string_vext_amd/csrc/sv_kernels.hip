@@ -509,6 +509,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
   const float o0sq = J.o0sqf, o1sq = J.o1sqf, in2 = J.in2f;
+  const float nk4 = -16.0f * o0sq;   // -4 k^2, k = 2|o0|
   const double rc2 = J.rc2;
   const float inv_rc2f = J.inv_rc2f, m2o0s = J.m2o0sf, m2o1s = J.m2o1sf;
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
@@ -521,6 +522,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t], lo2 = J.lo2f[t];
     const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
     const double c6s = J.c6s[t];   // shell loop scale
+    const float m128s6 = -128.0f * s6;
     const float s6s = J.s6sf[t];
     long long kept_far = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
@@ -584,8 +586,15 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
       // Each site's |r|^2 + o_s^2 is rounded once, here; an asymmetric guest's
       // second base is stored beside the entry rather than derived from the first.
       if (is_in) {
-        s_in[wv][lanes_below(min_)] = make_float4(rx, ry, rz, rr + o0sq);
-        if (!SYM) L.in1[wv][lanes_below(min_)] = rr + o1sq;
+        const float w = rr + o0sq;
+        if (SYM) {
+          // product form: (rx, ry, rz, 128 w) and 4 w^2
+          s_in[wv][lanes_below(min_)] = make_float4(rx, ry, rz, 128.0f * w);
+          L.in1[wv][lanes_below(min_)] = 4.0f * w * w;
+        } else {
+          s_in[wv][lanes_below(min_)] = make_float4(rx, ry, rz, w);
+          L.in1[wv][lanes_below(min_)] = rr + o1sq;
+        }
       }
       if (is_far) {
         // shell entries in units of rc^2 (see the shell loop)
@@ -611,7 +620,30 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
       // FP32 partials (one per site) cover at most kFlush atoms before they are
       // added into the FP64 accumulator, so a partial never grows past a few
       // hundred K and its rounding stays ~1e-5 K.
-      for (int q0 = 0; q0 < ni; q0 += kFlush) {
+      for (int q0 = 0; SYM && q0 < ni; q0 += kFlush) {
+        // Symmetric guest, both sites at once: d0,1 = w -/+ k t with k = 2|o|,
+        // P = 4 d0 d1 = 4 w^2 - 4 k^2 t^2, and with i_s = 1/d_s
+        //   i0^3 + i1^3 = 2w (4w^2 - 3P/4) 64/P^3,  i0^3 i1^3 = 64/P^3,
+        //   i0^6 + i1^6 = (i0^3 + i1^3)^2 - 2 i0^3 i1^3:
+        // one reciprocal per atom instead of two.
+        const int q1 = q0 + kFlush < ni ? q0 + kFlush : ni;
+        float acc = 0.f, acc1 = 0.f;
+#pragma unroll 4
+        for (int q = q0; q < q1; ++q) {
+          const float4 f = s_in[wv][q];
+          const float A = L.in1[wv][q];
+          const float t = f.x * ux + f.y * uy + f.z * uz;
+          const float P = fmaf(t * t, nk4, A);
+          const float ip = __builtin_amdgcn_rcpf(P);
+          const float ip3 = ip * ip * ip;
+          const float S3 = (f.w * fmaf(P, -0.75f, A)) * ip3;
+          acc = fmaf(S3, fmaf(s6, S3, -1.0f), acc);
+          acc1 = fmaf(m128s6, ip3, acc1);
+        }
+        E = fma(c6, (double)acc, E);
+        E = fma(c6, (double)acc1, E);
+      }
+      for (int q0 = 0; !SYM && q0 < ni; q0 += kFlush) {
         const int q1 = q0 + kFlush < ni ? q0 + kFlush : ni;
         float acc = 0.f, acc1 = 0.f;
 #pragma unroll 4
