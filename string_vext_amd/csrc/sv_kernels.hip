@@ -15,7 +15,8 @@
 //     sites against the compacted atoms, read from LDS by broadcast.
 //   * with r = img - c (uniform over the wave) and t = r.u (per lane), the
 //     two site distances are |r|^2 + o_s^2 - 2 o_s t: one dot product serves
-//     both sites.
+//     both sites; for a symmetric guest the atoms inside the cutoff for every
+//     orientation take both sites with one reciprocal (product form below).
 //
 // Mixed precision: the pair arithmetic is FP32 with the distance clamped to
 // at most rc^2 (for shell atoms d^2/rc^2 is clamped to [0, 1] by the clamp bit
