@@ -482,6 +482,7 @@ constexpr int kFlush = 16;
 struct MixedLds {
   float4 in[4][64];    // (rx, ry, rz, |r|^2 + o0^2)
   float4 far[4][64];
+  float4 out[4][64];   // outer shell (symmetric guests): (r, (|r|^2 + o^2) / rc^2)
   float in1[4][64];    // |r|^2 + o1^2 (asymmetric guests only)
   float far1[4][64];
   double4 near[4][64]; // FP64 (qx, qy, qz, |q|^2) of near-contact atoms
@@ -496,6 +497,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
                                            const double* __restrict__ list64) {
   float4 (*s_in)[64] = L.in;
   float4 (*s_far)[64] = L.far;
+  float4 (*s_out)[64] = L.out;
   const int lane = threadIdx.x & 63;
   const Job& J = *c.J;
   // The site field of a center is the same pair sum taken at the center
@@ -511,6 +513,12 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
   const float o0sq = J.o0sqf, o1sq = J.o1sqf, in2 = J.in2f;
   const float nk4 = -16.0f * o0sq;   // -4 k^2, k = 2|o0|
+  // Outer shell (symmetric guests): with |r|^2 >= rc^2 - o^2 the two site
+  // distances |r|^2 + o^2 -/+ 2|o| |t| cannot both be inside the cutoff, and
+  // the farther one never is; only the closer site is evaluated.  The margin
+  // covers the FP32 rounding of |r|^2.
+  const float out2 = SYM ? (float)((J.rc2 - (double)J.o0 * J.o0) * (1.0 + 1e-6)) : 3.0e38f;
+  const float m2abs = -fabsf(J.m2o0sf);
   const double rc2 = J.rc2;
   const float inv_rc2f = J.inv_rc2f, m2o0s = J.m2o0sf, m2o1s = J.m2o1sf;
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
@@ -525,7 +533,7 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     const double c6s = J.c6s[t];   // shell loop scale
     const float m128s6 = -128.0f * s6;
     const float s6s = J.s6sf[t];
-    long long kept_far = 0;
+    long long kept_far = 0, kept_out = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
     // Near-contact atoms are queued by list offset and evaluated when the queue
     // would overflow and at the end of the type: the compacting lanes stage
@@ -580,9 +588,11 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
       // inner: every orientation puts both sites strictly inside the cutoff, so
       // the pair needs no clamp
       const bool is_in = keep && !is_near && rr < in2;
-      const bool is_far = keep && !is_near && !is_in;
+      const bool is_out = SYM && keep && !is_near && !is_in && rr > out2;
+      const bool is_far = keep && !is_near && !is_in && !is_out;
       const unsigned long long min_ = __ballot(is_in);
       const unsigned long long mfar = __ballot(is_far);
+      const unsigned long long mout = SYM ? __ballot(is_out) : 0ull;
       const unsigned long long mnear = __ballot(is_near);
       // Each site's |r|^2 + o_s^2 is rounded once, here; an asymmetric guest's
       // second base is stored beside the entry rather than derived from the first.
@@ -602,6 +612,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, (rr + o0sq) * inv_rc2f);
         if (!SYM) L.far1[wv][lanes_below(mfar)] = (rr + o1sq) * inv_rc2f;
       }
+      if (SYM && is_out)
+        s_out[wv][lanes_below(mout)] = make_float4(rx, ry, rz, (rr + o0sq) * inv_rc2f);
       const int nn = __popcll(mnear);
       if (n_near + nn > 64) flush_near();
       if (is_near) L.near_idx[wv][n_near + lanes_below(mnear)] = (int)(idx - beg);
@@ -613,11 +625,12 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         const float is = __builtin_amdgcn_rcpf(ds);
         const float is6 = is * is * is;
         const float sf = is6 * fmaf(is6, s6, -1.0f);
-        Sf += is_far || is_in ? sf : 0.0f;
+        Sf += is_far || is_in || is_out ? sf : 0.0f;
       }
       wave_sync();
-      const int ni = __popcll(min_), nf = __popcll(mfar);
+      const int ni = __popcll(min_), nf = __popcll(mfar), no = SYM ? __popcll(mout) : 0;
       kept_far += ni + nf;
+      kept_out += no;
       // FP32 partials (one per site) cover at most kFlush atoms before they are
       // added into the FP64 accumulator, so a partial never grows past a few
       // hundred K and its rounding stays ~1e-5 K.
@@ -687,12 +700,29 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         E = fma(c6s, (double)acc, E);
         E = fma(c6s, (double)acc1, E);
       }
+      // Outer shell: the closer site only, |r|^2 + o^2 - 2|o||t| in units of
+      // rc^2 (|t| is a free input modifier of the fma).
+      for (int q0 = 0; SYM && q0 < no; q0 += kFlush) {
+        const int q1 = q0 + kFlush < no ? q0 + kFlush : no;
+        float acc = 0.f;
+#pragma unroll 4
+        for (int q = q0; q < q1; ++q) {
+          const float4 f = s_out[wv][q];
+          const float tdot = f.x * ux + f.y * uy + f.z * uz;
+          const float x = __builtin_amdgcn_fmed3f(fmaf(fabsf(tdot), m2abs, f.w), 0.0f, 1.0f);
+          const float i = __builtin_amdgcn_rcpf(x);
+          const float i3 = i * i * i;
+          acc = fmaf(i3, fmaf(i3, s6s, -1.0f), acc);
+        }
+        E = fma(c6s, (double)acc, E);
+      }
       wave_sync();
     }
     if (n_near) flush_near();
     // Every kept far pair was clamped to rc^2 at worst, contributing -shift.
-    E = fma((double)(2 * kept_far), sh, E);
-    S_shift = fma((double)kept_far, sh, S_shift);
+    // (one site per outer-shell atom: its other site is never inside)
+    E = fma((double)(2 * kept_far + kept_out), sh, E);
+    S_shift = fma((double)(kept_far + kept_out), sh, S_shift);
     S = fma(c6, (double)Sf, S);
   }
   write_pose<true>(c, E, do_site, lane == 0 ? S + S_shift : S);
