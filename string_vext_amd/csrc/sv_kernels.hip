@@ -506,21 +506,29 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const bool do_site = J.site32 != nullptr && c.group == 0;
 
   const double ux64 = J.axes64[3 * c.k], uy64 = J.axes64[3 * c.k + 1], uz64 = J.axes64[3 * c.k + 2];
-  const float ux = (float)ux64, uy = (float)uy64, uz = (float)uz64;
+  // A symmetric guest's FP32 loops take t' = r . (4|o| u): the inner loop's
+  // P = 4w^2 - (4|o| t)^2 is then one fma, and the shell constants absorb the
+  // scale.  (o = 0 makes t' = 0, and the unscaled constants are 0 too.)
+  const double k4d = SYM ? 4.0 * fabs(J.o0) : 1.0;
+  const bool scale_u = SYM && k4d > 0.0;
+  const double usc = SYM ? k4d : 1.0;
+  const float ux = (float)(ux64 * usc), uy = (float)(uy64 * usc), uz = (float)(uz64 * usc);
   const double cx = J.center_rel[3 * c.slot], cy = J.center_rel[3 * c.slot + 1],
                cz = J.center_rel[3 * c.slot + 2];
   const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
   const float o0sq = J.o0sqf, o1sq = J.o1sqf, in2 = J.in2f;
-  const float nk4 = -16.0f * o0sq;   // -4 k^2, k = 2|o0|
+
   // Outer shell (symmetric guests): with |r|^2 >= rc^2 - o^2 the two site
   // distances |r|^2 + o^2 -/+ 2|o| |t| cannot both be inside the cutoff, and
   // the farther one never is; only the closer site is evaluated.  The margin
   // covers the FP32 rounding of |r|^2.
   const float out2 = SYM ? (float)((J.rc2 - (double)J.o0 * J.o0) * (1.0 + 1e-6)) : 3.0e38f;
-  const float m2abs = -fabsf(J.m2o0sf);
+  const float m2abs = scale_u ? (float)(-0.5 / J.rc2) : -fabsf(J.m2o0sf);
   const double rc2 = J.rc2;
-  const float inv_rc2f = J.inv_rc2f, m2o0s = J.m2o0sf, m2o1s = J.m2o1sf;
+  const float inv_rc2f = J.inv_rc2f;
+  const float m2o0s = scale_u ? (float)(-copysign(0.5, J.o0) / J.rc2) : J.m2o0sf;
+  const float m2o1s = scale_u ? -m2o0s : J.m2o1sf;
   const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
   const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1, do1 = o1sqd - o0sqd;
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
@@ -646,8 +654,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         for (int q = q0; q < q1; ++q) {
           const float4 f = s_in[wv][q];
           const float A = L.in1[wv][q];
-          const float t = f.x * ux + f.y * uy + f.z * uz;
-          const float P = fmaf(t * t, nk4, A);
+          const float t = f.x * ux + f.y * uy + f.z * uz;   // 4|o| r.u
+          const float P = fmaf(-t, t, A);
           const float ip = __builtin_amdgcn_rcpf(P);
           const float ip3 = ip * ip * ip;
           const float S3 = (f.w * fmaf(P, -0.75f, A)) * ip3;
