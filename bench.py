@@ -80,10 +80,12 @@ def error_metrics(reference, candidate):
     diff = np.abs(reference - candidate)
     acc = (reference < 2000.0) & (np.abs(reference) < 1e4)
     rel = diff / np.maximum(np.abs(reference), 1.0)
+    rel100 = diff / np.maximum(np.abs(reference), 100.0)
     return {"accessible_poses": int(acc.sum()),
             "accessible_max_abs_error_K": float(diff[acc].max()),
             "accessible_rms_error_K": float(np.sqrt((diff[acc] ** 2).mean())),
-            "all_max_relative_error_floor1K": float(rel.max())}
+            "all_max_relative_error_floor1K": float(rel.max()),
+            "all_max_relative_error_floor100K": float(rel100.max())}
 
 
 def cpu_model():

@@ -480,11 +480,12 @@ __device__ __forceinline__ double pair64_in(double d2, double c12, double c6, do
 constexpr int kFlush = 16;
 
 struct MixedLds {
-  float4 in[4][64];    // (rx, ry, rz, |r|^2 + o0^2)
+  float4 in[4][64];    // (rx, ry, rz, |r|^2 + o0^2); symmetric guests: (r, 4 w^2)
   float4 far[4][64];
   float4 out[4][64];   // outer shell (symmetric guests): (r, (|r|^2 + o^2) / rc^2)
   float in1[4][64];    // |r|^2 + o1^2 (asymmetric guests only)
   float far1[4][64];
+  float2 inb[4][64];   // symmetric guests' inner entries: (128 w A, -96 w)
   double4 near[4][64]; // FP64 (qx, qy, qz, |q|^2) of near-contact atoms
   int near_idx[4][64]; // queued near-contact list entries (offset in the type segment)
 };
@@ -539,7 +540,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t], lo2 = J.lo2f[t];
     const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
     const double c6s = J.c6s[t];   // shell loop scale
-    const float m128s6 = -128.0f * s6;
+    const float k128s6 = 128.0f * s6;
+    const double c6s6 = c6 * (double)s6;
     const float s6s = J.s6sf[t];
     long long kept_far = 0, kept_out = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
@@ -580,6 +582,107 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
       wave_sync();
       n_near = 0;
     };
+    // The three FP32 classes queue their atoms in LDS across chunks, like the
+    // near-contact class: a queue is evaluated when the next chunk would
+    // overflow it and at the end of the type, so the orientation loops run
+    // over long runs of atoms rather than a few per chunk.  FP32 partials cover
+    // at most kFlush atoms before they are added into the FP64 accumulator, so
+    // a partial never grows past a few hundred K and its rounding stays ~1e-5 K.
+    int n_in = 0, n_far = 0, n_out = 0;
+    auto run_in = [&]() {
+      for (int q0 = 0; SYM && q0 < n_in; q0 += kFlush) {
+        // Symmetric guest, both sites at once: d0,1 = w -/+ k t with k = 2|o|,
+        // P = 4 d0 d1 = A - t'^2 with A = 4 w^2 and t' = 2 k t; a = 1/P, i_s = 1/d_s:
+        //   S3 = i0^3 + i1^3 = 128 w (A - 3P/4) a^3 = a^2 (B a + C),
+        //   i0^3 i1^3 = 64 a^3,  i0^6 + i1^6 = S3^2 - 128 a^3,
+        // with B = 128 w A and C = -96 w stored per atom; both sites together are
+        //   s6 (i0^6 + i1^6) - S3 = s6 S3^2 - a^2 (B a + C + 128 s6 a):
+        // one reciprocal and ten VALU operations per atom.
+        const int q1 = q0 + kFlush < n_in ? q0 + kFlush : n_in;
+        float acc = 0.f, acc1 = 0.f;
+#pragma unroll 4
+        for (int q = q0; q < q1; ++q) {
+          const float4 f = s_in[wv][q];
+          const float2 g = L.inb[wv][q];
+          const float t = f.x * ux + f.y * uy + f.z * uz;   // 4|o| r.u
+          const float P = fmaf(-t, t, f.w);
+          const float a = __builtin_amdgcn_rcpf(P);
+          const float a2 = a * a;
+          const float g1 = fmaf(g.x, a, g.y);
+          const float S3 = a2 * g1;
+          acc = fmaf(S3, S3, acc);                        // times c6 s6
+          acc1 = fmaf(-a2, fmaf(k128s6, a, g1), acc1);   // times c6
+        }
+        E = fma(c6s6, (double)acc, E);
+        E = fma(c6, (double)acc1, E);
+      }
+      for (int q0 = 0; !SYM && q0 < n_in; q0 += kFlush) {
+        const int q1 = q0 + kFlush < n_in ? q0 + kFlush : n_in;
+        float acc = 0.f, acc1 = 0.f;
+#pragma unroll 4
+        for (int q = q0; q < q1; ++q) {
+          const float4 f = s_in[wv][q];
+          const float tdot = f.x * ux + f.y * uy + f.z * uz;
+          const float d0 = fmaf(tdot, m2o0, f.w);
+          const float d1 = fmaf(tdot, m2o1, SYM ? f.w : L.in1[wv][q]);
+          const float i0 = __builtin_amdgcn_rcpf(d0);
+          const float i1 = __builtin_amdgcn_rcpf(d1);
+          const float i06 = i0 * i0 * i0;
+          const float i16 = i1 * i1 * i1;
+          acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
+          acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
+        }
+        E = fma(c6, (double)acc, E);
+        E = fma(c6, (double)acc1, E);
+      }
+      n_in = 0;
+    };
+    // Shell atoms work in units of rc^2: x = d^2 / rc^2 is clamped to [0, 1]
+    // by the clamp bit of the fma that forms it (med3(., 0, 1) folds into it),
+    // and c6 / rc^6, sigma^6 / rc^6 absorb the scale.  A pair beyond the cutoff
+    // gives x = 1, i.e. exactly the clamped -shift of the unscaled form.
+    auto run_far = [&]() {
+      for (int q0 = 0; q0 < n_far; q0 += kFlush) {
+        const int q1 = q0 + kFlush < n_far ? q0 + kFlush : n_far;
+        float acc = 0.f, acc1 = 0.f;
+#pragma unroll 4
+        for (int q = q0; q < q1; ++q) {
+          const float4 f = s_far[wv][q];
+          const float tdot = f.x * ux + f.y * uy + f.z * uz;
+          const float x0 = __builtin_amdgcn_fmed3f(fmaf(tdot, m2o0s, f.w), 0.0f, 1.0f);
+          const float x1 =
+              __builtin_amdgcn_fmed3f(fmaf(tdot, m2o1s, SYM ? f.w : L.far1[wv][q]), 0.0f, 1.0f);
+          const float i0 = __builtin_amdgcn_rcpf(x0);
+          const float i1 = __builtin_amdgcn_rcpf(x1);
+          const float i06 = i0 * i0 * i0;
+          const float i16 = i1 * i1 * i1;
+          acc = fmaf(i06, fmaf(i06, s6s, -1.0f), acc);
+          acc1 = fmaf(i16, fmaf(i16, s6s, -1.0f), acc1);
+        }
+        E = fma(c6s, (double)acc, E);
+        E = fma(c6s, (double)acc1, E);
+      }
+      n_far = 0;
+    };
+    // Outer shell: the closer site only, |r|^2 + o^2 - 2|o||t| in units of
+    // rc^2 (|t| is a free input modifier of the fma).
+    auto run_out = [&]() {
+      for (int q0 = 0; SYM && q0 < n_out; q0 += kFlush) {
+        const int q1 = q0 + kFlush < n_out ? q0 + kFlush : n_out;
+        float acc = 0.f;
+#pragma unroll 4
+        for (int q = q0; q < q1; ++q) {
+          const float4 f = s_out[wv][q];
+          const float tdot = f.x * ux + f.y * uy + f.z * uz;
+          const float x = __builtin_amdgcn_fmed3f(fmaf(fabsf(tdot), m2abs, f.w), 0.0f, 1.0f);
+          const float i = __builtin_amdgcn_rcpf(x);
+          const float i3 = i * i * i;
+          acc = fmaf(i3, fmaf(i3, s6s, -1.0f), acc);
+        }
+        E = fma(c6s, (double)acc, E);
+      }
+      n_out = 0;
+    };
     for (long long base = beg; base < end; base += 64) {
       const long long idx = base + lane;
       const bool valid = idx < end;
@@ -602,30 +705,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
       const unsigned long long mfar = __ballot(is_far);
       const unsigned long long mout = SYM ? __ballot(is_out) : 0ull;
       const unsigned long long mnear = __ballot(is_near);
-      // Each site's |r|^2 + o_s^2 is rounded once, here; an asymmetric guest's
-      // second base is stored beside the entry rather than derived from the first.
-      if (is_in) {
-        const float w = rr + o0sq;
-        if (SYM) {
-          // product form: (rx, ry, rz, 128 w) and 4 w^2
-          s_in[wv][lanes_below(min_)] = make_float4(rx, ry, rz, 128.0f * w);
-          L.in1[wv][lanes_below(min_)] = 4.0f * w * w;
-        } else {
-          s_in[wv][lanes_below(min_)] = make_float4(rx, ry, rz, w);
-          L.in1[wv][lanes_below(min_)] = rr + o1sq;
-        }
-      }
-      if (is_far) {
-        // shell entries in units of rc^2 (see the shell loop)
-        s_far[wv][lanes_below(mfar)] = make_float4(rx, ry, rz, (rr + o0sq) * inv_rc2f);
-        if (!SYM) L.far1[wv][lanes_below(mfar)] = (rr + o1sq) * inv_rc2f;
-      }
-      if (SYM && is_out)
-        s_out[wv][lanes_below(mout)] = make_float4(rx, ry, rz, (rr + o0sq) * inv_rc2f);
+      const int ni = __popcll(min_), nf = __popcll(mfar), no = SYM ? __popcll(mout) : 0;
       const int nn = __popcll(mnear);
-      if (n_near + nn > 64) flush_near();
-      if (is_near) L.near_idx[wv][n_near + lanes_below(mnear)] = (int)(idx - beg);
-      n_near += nn;
       if (do_site) {
         // far candidates: clamped FP32 term (the shift is added per kept entry
         // below); near candidates get the FP64 reference term when flushed.
@@ -635,97 +716,45 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         const float sf = is6 * fmaf(is6, s6, -1.0f);
         Sf += is_far || is_in || is_out ? sf : 0.0f;
       }
-      wave_sync();
-      const int ni = __popcll(min_), nf = __popcll(mfar), no = SYM ? __popcll(mout) : 0;
+      if (n_in + ni > 64) { run_in(); wave_sync(); }
+      if (n_far + nf > 64) { run_far(); wave_sync(); }
+      if (SYM && n_out + no > 64) { run_out(); wave_sync(); }
+      if (n_near + nn > 64) flush_near();
+      // Each site's |r|^2 + o_s^2 is rounded once, here; an asymmetric guest's
+      // second base is stored beside the entry rather than derived from the first.
+      if (is_in) {
+        const int slot = n_in + lanes_below(min_);
+        const float w = rr + o0sq;
+        if (SYM) {
+          // product form: (rx, ry, rz, A = 4 w^2) and (128 w A, -96 w)
+          const float A = 4.0f * w * w, fw = 128.0f * w;
+          s_in[wv][slot] = make_float4(rx, ry, rz, A);
+          L.inb[wv][slot] = make_float2(fw * A, -0.75f * fw);
+        } else {
+          s_in[wv][slot] = make_float4(rx, ry, rz, w);
+          L.in1[wv][slot] = rr + o1sq;
+        }
+      }
+      if (is_far) {
+        // shell entries in units of rc^2 (see the shell loop)
+        const int slot = n_far + lanes_below(mfar);
+        s_far[wv][slot] = make_float4(rx, ry, rz, (rr + o0sq) * inv_rc2f);
+        if (!SYM) L.far1[wv][slot] = (rr + o1sq) * inv_rc2f;
+      }
+      if (SYM && is_out)
+        s_out[wv][n_out + lanes_below(mout)] = make_float4(rx, ry, rz, (rr + o0sq) * inv_rc2f);
+      if (is_near) L.near_idx[wv][n_near + lanes_below(mnear)] = (int)(idx - beg);
+      n_in += ni;
+      n_far += nf;
+      n_out += no;
+      n_near += nn;
       kept_far += ni + nf;
       kept_out += no;
-      // FP32 partials (one per site) cover at most kFlush atoms before they are
-      // added into the FP64 accumulator, so a partial never grows past a few
-      // hundred K and its rounding stays ~1e-5 K.
-      for (int q0 = 0; SYM && q0 < ni; q0 += kFlush) {
-        // Symmetric guest, both sites at once: d0,1 = w -/+ k t with k = 2|o|,
-        // P = 4 d0 d1 = 4 w^2 - 4 k^2 t^2, and with i_s = 1/d_s
-        //   i0^3 + i1^3 = 2w (4w^2 - 3P/4) 64/P^3,  i0^3 i1^3 = 64/P^3,
-        //   i0^6 + i1^6 = (i0^3 + i1^3)^2 - 2 i0^3 i1^3:
-        // one reciprocal per atom instead of two.
-        const int q1 = q0 + kFlush < ni ? q0 + kFlush : ni;
-        float acc = 0.f, acc1 = 0.f;
-#pragma unroll 4
-        for (int q = q0; q < q1; ++q) {
-          const float4 f = s_in[wv][q];
-          const float A = L.in1[wv][q];
-          const float t = f.x * ux + f.y * uy + f.z * uz;   // 4|o| r.u
-          const float P = fmaf(-t, t, A);
-          const float ip = __builtin_amdgcn_rcpf(P);
-          const float ip3 = ip * ip * ip;
-          const float S3 = (f.w * fmaf(P, -0.75f, A)) * ip3;
-          acc = fmaf(S3, fmaf(s6, S3, -1.0f), acc);
-          acc1 = fmaf(m128s6, ip3, acc1);
-        }
-        E = fma(c6, (double)acc, E);
-        E = fma(c6, (double)acc1, E);
-      }
-      for (int q0 = 0; !SYM && q0 < ni; q0 += kFlush) {
-        const int q1 = q0 + kFlush < ni ? q0 + kFlush : ni;
-        float acc = 0.f, acc1 = 0.f;
-#pragma unroll 4
-        for (int q = q0; q < q1; ++q) {
-          const float4 f = s_in[wv][q];
-          const float tdot = f.x * ux + f.y * uy + f.z * uz;
-          const float d0 = fmaf(tdot, m2o0, f.w);
-          const float d1 = fmaf(tdot, m2o1, SYM ? f.w : L.in1[wv][q]);
-          const float i0 = __builtin_amdgcn_rcpf(d0);
-          const float i1 = __builtin_amdgcn_rcpf(d1);
-          const float i06 = i0 * i0 * i0;
-          const float i16 = i1 * i1 * i1;
-          acc = fmaf(i06, fmaf(i06, s6, -1.0f), acc);
-          acc1 = fmaf(i16, fmaf(i16, s6, -1.0f), acc1);
-        }
-        E = fma(c6, (double)acc, E);
-        E = fma(c6, (double)acc1, E);
-      }
-      // Shell atoms work in units of rc^2: x = d^2 / rc^2 is clamped to [0, 1]
-      // by the clamp bit of the fma that forms it (med3(., 0, 1) folds into it),
-      // and c6 / rc^6, sigma^6 / rc^6 absorb the scale.  A pair beyond the cutoff
-      // gives x = 1, i.e. exactly the clamped -shift of the unscaled form.
-      for (int q0 = 0; q0 < nf; q0 += kFlush) {
-        const int q1 = q0 + kFlush < nf ? q0 + kFlush : nf;
-        float acc = 0.f, acc1 = 0.f;
-#pragma unroll 4
-        for (int q = q0; q < q1; ++q) {
-          const float4 f = s_far[wv][q];
-          const float tdot = f.x * ux + f.y * uy + f.z * uz;
-          const float x0 = __builtin_amdgcn_fmed3f(fmaf(tdot, m2o0s, f.w), 0.0f, 1.0f);
-          const float x1 =
-              __builtin_amdgcn_fmed3f(fmaf(tdot, m2o1s, SYM ? f.w : L.far1[wv][q]), 0.0f, 1.0f);
-          const float i0 = __builtin_amdgcn_rcpf(x0);
-          const float i1 = __builtin_amdgcn_rcpf(x1);
-          const float i06 = i0 * i0 * i0;
-          const float i16 = i1 * i1 * i1;
-          acc = fmaf(i06, fmaf(i06, s6s, -1.0f), acc);
-          acc1 = fmaf(i16, fmaf(i16, s6s, -1.0f), acc1);
-        }
-        E = fma(c6s, (double)acc, E);
-        E = fma(c6s, (double)acc1, E);
-      }
-      // Outer shell: the closer site only, |r|^2 + o^2 - 2|o||t| in units of
-      // rc^2 (|t| is a free input modifier of the fma).
-      for (int q0 = 0; SYM && q0 < no; q0 += kFlush) {
-        const int q1 = q0 + kFlush < no ? q0 + kFlush : no;
-        float acc = 0.f;
-#pragma unroll 4
-        for (int q = q0; q < q1; ++q) {
-          const float4 f = s_out[wv][q];
-          const float tdot = f.x * ux + f.y * uy + f.z * uz;
-          const float x = __builtin_amdgcn_fmed3f(fmaf(fabsf(tdot), m2abs, f.w), 0.0f, 1.0f);
-          const float i = __builtin_amdgcn_rcpf(x);
-          const float i3 = i * i * i;
-          acc = fmaf(i3, fmaf(i3, s6s, -1.0f), acc);
-        }
-        E = fma(c6s, (double)acc, E);
-      }
       wave_sync();
     }
+    if (n_in) run_in();
+    if (n_far) run_far();
+    if (SYM && n_out) run_out();
     if (n_near) flush_near();
     // Every kept far pair was clamped to rc^2 at worst, contributing -shift.
     // (one site per outer-shell atom: its other site is never inside)

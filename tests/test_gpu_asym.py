@@ -88,7 +88,7 @@ def test_batch_with_symmetric_and_asymmetric_guests(oracle, case, persistent):
     batch.run(sync=True)
     for j in range(len(jobs)):
         m = oracle.error_metrics(refs[j], batch.fetch("energy", j))
-        assert m["all_max_relative_error_floor1K"] < 1e-5
+        assert m["all_max_relative_error_floor100K"] < 1e-5
         if m["accessible_poses"]:
             assert m["accessible_max_abs_error_K"] < 1e-3
 

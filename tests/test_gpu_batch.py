@@ -52,7 +52,8 @@ def test_batch_matches_oracle_and_tiles_do_not_change_results(oracle, ragged):
     sizes, ks = [5, 4, 5], [64, 64, 64]
     ref = [oracle.reuse_energy(c, *regular_pose_grid(s, k)) for c, s, k in zip(cases, sizes, ks)]
     for tc in (16, 64, 256):
-        for precision, tol in (("fp64", 1e-10), ("mixed", 1e-5)):
+        # mixed: 1e-5 relative with a 100 K floor (at most 1e-3 K below 100 K)
+        for precision, tol, floor in (("fp64", 1e-10, 1.0), ("mixed", 1e-5, 100.0)):
             jobs = _jobs(cases, sizes, ks, tile_centers=tc)
             batch = Batch(jobs, precision=precision, outputs=("energy",))
             if precision == "mixed":
@@ -60,7 +61,7 @@ def test_batch_matches_oracle_and_tiles_do_not_change_results(oracle, ragged):
             batch.run(sync=True)
             for j in range(len(jobs)):
                 got = batch.fetch("energy", j)
-                rel = np.abs(got - ref[j]) / np.maximum(np.abs(ref[j]), 1.0)
+                rel = np.abs(got - ref[j]) / np.maximum(np.abs(ref[j]), floor)
                 assert rel.max() < tol, (tc, precision, j, rel.max())
                 if precision == "mixed":
                     m = oracle.error_metrics(ref[j], got)

@@ -99,9 +99,11 @@ def test_field_matches_reference_golden(name, precision):
     axes = axes / np.linalg.norm(axes, axis=1, keepdims=True)
     plan = Plan.from_case(case)
     out = plan.field(plan.poses(centers, axes), precision=precision)["energy"]
-    rel = np.abs(out - ref) / np.maximum(np.abs(ref), 1.0)
     # FP64: the plain GPU form within 1e-6 relative (1 K floor) of the reference CPU
-    # kernel -- in practice rounding-level; mixed: the stated mixed tolerance
+    # kernel -- in practice rounding-level; mixed: the stated mixed tolerance,
+    # 1e-5 relative with a 100 K floor (at most 1e-3 K below 100 K)
+    floor = 1.0 if precision == "fp64" else 100.0
+    rel = np.abs(out - ref) / np.maximum(np.abs(ref), floor)
     assert rel.max() < (1e-12 if precision == "fp64" else 1e-5), rel.max()
 
 
@@ -120,7 +122,7 @@ def test_small_field_all_channels_match_oracle(oracle, small_case, precision):
         assert m["all_max_relative_error_floor1K"] < 1e-12
     else:
         assert m["accessible_max_abs_error_K"] < 1e-3
-        assert m["all_max_relative_error_floor1K"] < 1e-5
+        assert m["all_max_relative_error_floor100K"] < 1e-5
     np.testing.assert_allclose(out["orientation"].reshape(ref["orientation_K"].shape),
                                ref["orientation_K"], rtol=1e-6, atol=1e-3)
     np.testing.assert_allclose(out["marginal"].reshape(ref["marginal_K"].shape),
@@ -211,4 +213,4 @@ def test_short_cutoff_near_contact_pairs_reach_the_cutoff(oracle, small_case, pr
         # no poses fall under the accessible mask; the relative bound holds)
         if m["accessible_poses"]:
             assert m["accessible_max_abs_error_K"] < 1e-3
-        assert m["all_max_relative_error_floor1K"] < 1e-5
+        assert m["all_max_relative_error_floor100K"] < 1e-5
