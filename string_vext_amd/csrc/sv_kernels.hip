@@ -54,6 +54,17 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// Wave-uniform values loaded through pointers (vector loads) or computed by
+// VALU land in vector registers; readfirstlane moves them to scalar registers.
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform_d(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8).
 // Map hardware block b to a logical block so that runs of kXcdRun consecutive
@@ -514,9 +525,9 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const bool scale_u = SYM && k4d > 0.0;
   const double usc = SYM ? k4d : 1.0;
   const float ux = (float)(ux64 * usc), uy = (float)(uy64 * usc), uz = (float)(uz64 * usc);
-  const double cx = J.center_rel[3 * c.slot], cy = J.center_rel[3 * c.slot + 1],
-               cz = J.center_rel[3 * c.slot + 2];
-  const float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
+  const double cx = uniform_d(J.center_rel[3 * c.slot]), cy = uniform_d(J.center_rel[3 * c.slot + 1]),
+               cz = uniform_d(J.center_rel[3 * c.slot + 2]);
+  const float cxf = uniform_f((float)cx), cyf = uniform_f((float)cy), czf = uniform_f((float)cz);
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
   const float o0sq = J.o0sqf, o1sq = J.o1sqf, in2 = J.in2f;
 
@@ -524,25 +535,28 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   // distances |r|^2 + o^2 -/+ 2|o| |t| cannot both be inside the cutoff, and
   // the farther one never is; only the closer site is evaluated.  The margin
   // covers the FP32 rounding of |r|^2.
-  const float out2 = SYM ? (float)((J.rc2 - (double)J.o0 * J.o0) * (1.0 + 1e-6)) : 3.0e38f;
-  const float m2abs = scale_u ? (float)(-0.5 / J.rc2) : -fabsf(J.m2o0sf);
+  const float out2 = uniform_f(SYM ? (float)((J.rc2 - (double)J.o0 * J.o0) * (1.0 + 1e-6)) : 3.0e38f);
+  const float m2abs = uniform_f(scale_u ? (float)(-0.5 / J.rc2) : -fabsf(J.m2o0sf));
   const double rc2 = J.rc2;
   const float inv_rc2f = J.inv_rc2f;
-  const float m2o0s = scale_u ? (float)(-copysign(0.5, J.o0) / J.rc2) : J.m2o0sf;
-  const float m2o1s = scale_u ? -m2o0s : J.m2o1sf;
-  const double m2o0d = -2.0 * J.o0, m2o1d = -2.0 * J.o1;
-  const double o0sqd = J.o0 * J.o0, o1sqd = J.o1 * J.o1, do1 = o1sqd - o0sqd;
+  const float m2o0s = uniform_f(scale_u ? (float)(-copysign(0.5, J.o0) / J.rc2) : J.m2o0sf);
+  const float m2o1s = uniform_f(scale_u ? -m2o0s : J.m2o1sf);
+  const double m2o0d = uniform_d(-2.0 * J.o0), m2o1d = uniform_d(-2.0 * J.o1);
+  const double o0sqd = uniform_d(J.o0 * J.o0), o1sqd = uniform_d(J.o1 * J.o1);
+  const double do1 = uniform_d(o1sqd - o0sqd);
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
 
   double E = 0.0, S = 0.0, S_shift = 0.0;
   for (int t = 0; t < J.n_types; ++t) {
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
-    const float s6 = J.s6f[t], fl2f = J.floor2f[t], near2 = J.near2f[t], lo2 = J.lo2f[t];
-    const double c6 = J.c6[t], c12 = J.c12[t], sh = J.shift[t], fl2 = J.floor2[t];
-    const double c6s = J.c6s[t];   // shell loop scale
-    const float k128s6 = 128.0f * s6;
-    const double c6s6 = c6 * (double)s6;
-    const float s6s = J.s6sf[t];
+    const float s6 = uniform_f(J.s6f[t]), fl2f = uniform_f(J.floor2f[t]);
+    const float near2 = uniform_f(J.near2f[t]), lo2 = uniform_f(J.lo2f[t]);
+    const double c6 = uniform_d(J.c6[t]), c12 = uniform_d(J.c12[t]), sh = uniform_d(J.shift[t]);
+    const double fl2 = uniform_d(J.floor2[t]);
+    const double c6s = uniform_d(J.c6s[t]);   // shell loop scale
+    const float k128s6 = uniform_f(128.0f * s6);
+    const double c6s6 = uniform_d(c6 * (double)s6);
+    const float s6s = uniform_f(J.s6sf[t]);
     long long kept_far = 0, kept_out = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
     // Near-contact atoms are queued by list offset and evaluated when the queue
