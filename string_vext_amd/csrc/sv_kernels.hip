@@ -706,19 +706,27 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
         rx = e.x - cxf; ry = e.y - cyf; rz = e.z - czf;
         rr = rx * rx + ry * ry + rz * rz;
       }
+      // (bitwise, not short-circuit, so that each class is a lane mask built
+      // from the compare masks with scalar logic)
       const bool keep = rr < keep2f;
       // Atoms some orientation brings within the contact floor are always FP64,
       // so the FP32 classes never need the floor clamp.
-      const bool is_near = keep && (rr < near2 || rr <= lo2);
+      const bool is_near = keep & ((rr < near2) | (rr <= lo2));
       // inner: every orientation puts both sites strictly inside the cutoff, so
       // the pair needs no clamp
-      const bool is_in = keep && !is_near && rr < in2;
-      const bool is_out = SYM && keep && !is_near && !is_in && rr > out2;
-      const bool is_far = keep && !is_near && !is_in && !is_out;
-      const unsigned long long min_ = __ballot(is_in);
-      const unsigned long long mfar = __ballot(is_far);
-      const unsigned long long mout = SYM ? __ballot(is_out) : 0ull;
-      const unsigned long long mnear = __ballot(is_near);
+      const bool is_in = keep & !is_near & (rr < in2);
+      const bool is_out = SYM & keep & !is_near & !is_in & (rr > out2);
+      const bool is_far = keep & !is_near & !is_in & !is_out;
+      // The class masks come from ballots of the plain compares and scalar
+      // logic (a ballot of a combined predicate costs two VALU operations).
+      const unsigned long long b_keep = __ballot(rr < keep2f);
+      const unsigned long long b_near = __ballot(rr < near2) | __ballot(rr <= lo2);
+      const unsigned long long b_in = __ballot(rr < in2);
+      const unsigned long long b_out = SYM ? __ballot(rr > out2) : 0ull;
+      const unsigned long long mnear = b_keep & b_near;
+      const unsigned long long min_ = b_keep & ~b_near & b_in;
+      const unsigned long long mout = b_keep & ~b_near & ~b_in & b_out;
+      const unsigned long long mfar = b_keep & ~b_near & ~b_in & ~b_out;
       const int ni = __popcll(min_), nf = __popcll(mfar), no = SYM ? __popcll(mout) : 0;
       const int nn = __popcll(mnear);
       if (do_site) {
