@@ -497,7 +497,10 @@ struct MixedLds {
   float in1[4][64];    // |r|^2 + o1^2 (asymmetric guests only)
   float far1[4][64];
   float2 inb[4][64];   // symmetric guests' inner entries: (128 w A, -96 w)
-  double4 near[4][64]; // FP64 (qx, qy, qz, |q|^2) of near-contact atoms
+  double4 near[4][32]; // FP64 (qx, qy, qz, |q|^2 + o0^2) of near-contact atoms, 32 at a time
+  // symmetric guests: (w^2, 8 w^3, -6 w, 1) with w = |q|^2 + o^2 for an atom
+  // that takes the product form, (., ., ., 0) otherwise
+  double4 nearb[4][32];
   int near_idx[4][64]; // queued near-contact list entries (offset in the type segment)
 };
 
@@ -542,6 +545,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const float m2o0s = uniform_f(scale_u ? (float)(-copysign(0.5, J.o0) / J.rc2) : J.m2o0sf);
   const float m2o1s = uniform_f(scale_u ? -m2o0s : J.m2o1sf);
   const double m2o0d = uniform_d(-2.0 * J.o0), m2o1d = uniform_d(-2.0 * J.o1);
+  // near-contact product form: symmetric guest, every near-contact site inside
+  const bool near_pf = SYM && J.near_inside;
   const double o0sqd = uniform_d(J.o0 * J.o0), o1sqd = uniform_d(J.o1 * J.o1);
   const double do1 = uniform_d(o1sqd - o0sqd);
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
@@ -553,6 +558,8 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     const float near2 = uniform_f(J.near2f[t]), lo2 = uniform_f(J.lo2f[t]);
     const double c6 = uniform_d(J.c6[t]), c12 = uniform_d(J.c12[t]), sh = uniform_d(J.shift[t]);
     const double fl2 = uniform_d(J.floor2[t]);
+    const double m2c12 = uniform_d(-2.0 * c12);
+    const double lo2d = (double)lo2;
     const double c6s = uniform_d(J.c6s[t]);   // shell loop scale
     const float k128s6 = uniform_f(128.0f * s6);
     const double c6s6 = uniform_d(c6 * (double)s6);
@@ -566,31 +573,52 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
     // than once per chunk keeps the FP64 code off most chunks.
     int n_near = 0;
     auto flush_near = [&]() {
-      wave_sync();
-      if (lane < n_near) {
-        const long long id = beg + L.near_idx[wv][lane];
-        const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
-                     qz = list64[3 * id + 2] - cz;
-        const double RR = qx * qx + qy * qy + qz * qz;
-        // w = |q|^2 + o0^2; the second site's base is w + (o1^2 - o0^2)
-        L.near[wv][lane] = make_double4(qx, qy, qz, RR + o0sqd);
-        if (do_site) S += pair64_d2(RR, rc2, c12, c6, sh, fl2);
-      }
-      wave_sync();
-      // FP64 near-contact pairs: one dot product serves both sites.
-      if (J.near_inside) {
-        for (int q = 0; q < n_near; ++q) {
-          const double4 d = L.near[wv][q];
-          const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
-          E += pair64_in(fma(tt, m2o0d, d.w), c12, c6, sh, fl2);
-          E += pair64_in(fma(tt, m2o1d, SYM ? d.w : d.w + do1), c12, c6, sh, fl2);
+      for (int h = 0; h < n_near; h += 32) {
+        const int m = n_near - h < 32 ? n_near - h : 32;
+        wave_sync();
+        // Stage up to 32 atoms lane-parallel.  With a symmetric guest and every
+        // near-contact site inside the cutoff, an atom that no orientation
+        // brings within the contact floor (|q| above the floor reach) takes
+        // both sites in one product-form term; the others take the per-site
+        // form with the floor.
+        if (lane < m) {
+          const long long id = beg + L.near_idx[wv][h + lane];
+          const double qx = list64[3 * id] - cx, qy = list64[3 * id + 1] - cy,
+                       qz = list64[3 * id + 2] - cz;
+          const double RR = qx * qx + qy * qy + qz * qz;
+          // w = |q|^2 + o0^2; the second site's base is w + (o1^2 - o0^2)
+          const double w = RR + o0sqd;
+          L.near[wv][lane] = make_double4(qx, qy, qz, w);
+          if (near_pf) {
+            const double w2 = w * w;
+            L.nearb[wv][lane] = make_double4(w2, 8.0 * w * w2, -6.0 * w, RR > lo2d ? 1.0 : 0.0);
+          }
+          if (do_site) S += pair64_d2(RR, rc2, c12, c6, sh, fl2);
         }
-      } else {
-        for (int q = 0; q < n_near; ++q) {
+        wave_sync();
+        for (int q = 0; q < m; ++q) {
           const double4 d = L.near[wv][q];
-          const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
-          E += pair64_d2(fma(tt, m2o0d, d.w), rc2, c12, c6, sh, fl2);
-          E += pair64_d2(fma(tt, m2o1d, SYM ? d.w : d.w + do1), rc2, c12, c6, sh, fl2);
+          if (near_pf && L.nearb[wv][q].w != 0.0) {
+            // Product form (both sites): d0,1 = w -/+ t' with t' = 2 o q.u,
+            // Q = d0 d1, a = 1/Q, S3 = 1/d0^3 + 1/d1^3 = a^2 (8 w^3 a - 6 w),
+            // 1/(d0 d1)^3 = a^3, 1/d0^6 + 1/d1^6 = S3^2 - 2 a^3.
+            const double4 b = L.nearb[wv][q];
+            const double tt = (d.x * ux64 + d.y * uy64 + d.z * uz64) * m2o0d;
+            const double a = inv_fast(fma(-tt, tt, b.x));
+            const double a2 = a * a;
+            const double S3 = a2 * fma(b.y, a, b.z);
+            E = fma(S3, fma(c12, S3, -c6), E);
+            E = fma(m2c12, a2 * a, fma(2.0, sh, E));
+          } else if (J.near_inside) {
+            // FP64 near-contact pairs: one dot product serves both sites.
+            const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
+            E += pair64_in(fma(tt, m2o0d, d.w), c12, c6, sh, fl2);
+            E += pair64_in(fma(tt, m2o1d, SYM ? d.w : d.w + do1), c12, c6, sh, fl2);
+          } else {
+            const double tt = d.x * ux64 + d.y * uy64 + d.z * uz64;
+            E += pair64_d2(fma(tt, m2o0d, d.w), rc2, c12, c6, sh, fl2);
+            E += pair64_d2(fma(tt, m2o1d, SYM ? d.w : d.w + do1), rc2, c12, c6, sh, fl2);
+          }
         }
       }
       wave_sync();
