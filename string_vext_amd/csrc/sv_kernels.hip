@@ -72,7 +72,7 @@ __device__ __forceinline__ double uniform_d(double v) {
 // its L2, while the runs rotate over the XCDs: every XCD gets an even share of
 // each part of the work range, so batches sorted by cost stay balanced.  The
 // last incomplete round maps to itself.
-constexpr long long kXcdRun = 16;
+constexpr long long kXcdRun = 64;
 __device__ __forceinline__ long long xcd_block(long long b, long long nb) {
   const long long round = 8 * kXcdRun;
   if (b >= nb / round * round) return b;
