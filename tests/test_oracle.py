@@ -224,3 +224,16 @@ def test_pose_gradient_matches_central_differences(oracle):
         em = oracle.pose_grad(case, centers, axes - dc, setup=setup)[0]
         fd = (ep - em) / (2 * h)
         np.testing.assert_allclose(gu[:, k], fd, rtol=1e-5, atol=1e-4 * (1 + np.abs(fd).max()))
+
+
+def test_error_metrics_floors(oracle):
+    """The 1 K floor is the reference probe's (exact_neighbor_reuse_probe.py:73-83);
+    the 100 K floor turns a 1e-5 bound into 1e-3 K absolute below 100 K."""
+    ref = np.array([-5.0, 0.5, 250.0, 3.0e4])
+    got = ref + np.array([5e-4, 2e-5, 1e-3, 0.2])
+    m = oracle.error_metrics(ref, got)
+    assert m["accessible_poses"] == 3
+    assert m["accessible_max_abs_error_K"] == pytest.approx(1e-3)
+    assert m["all_max_relative_error_floor1K"] == pytest.approx(1e-4)     # 5e-4 / 5
+    assert m["all_max_relative_error_floor100K"] == pytest.approx(2e-5 / 3)  # 0.2 / 3e4
+    assert m["all_max_abs_error_K"] == pytest.approx(0.2)
