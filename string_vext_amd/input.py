@@ -1,11 +1,20 @@
-"""Parser of the native String ``input.dat`` format for neutral two-site guests.
+"""Reader and writer of the native String ``input.dat`` format for neutral
+two-site guests.
 
-Mirrors ``string_vext.input.parse_string_input`` (reference
-src/string_vext/input.py:11-57): values follow header lines located by prefix;
-framework rows are ``ID sigma epsilon charge mass fa fb fc name``, guest rows
-``x y z epsilon sigma charge mass``; charged or malformed inputs are rejected;
-the guest is re-expressed as a canonical body on the x axis about its centre of
-mass.
+The format (reference src/string_vext/input.py:11-57, read positionally by
+the String solver, native/string/source_code/RC_main.cu:282-387) is a list of
+header lines, each followed by its values:
+
+    La Lb Lc ...           a b c (A)
+    Alpha Beta Gamma       alpha beta gamma (deg)
+    cutoff(A) ...          cutoff, FH signal, guest mass (g/mol), temperature (K), ...
+    Number of sites        n, then a column header, then n rows  x y z eps sigma q m
+    Number of atoms        n, then a column header, then n rows  ID sigma eps q m fa fb fc name
+
+``parse_string_input`` accepts a neutral guest of exactly two sites and a
+neutral framework with IDs 1..n, positive LJ parameters, masses, cutoff and
+temperature, and a total mass equal to the site masses (relative 1e-7).  The
+guest is returned as a canonical body on the x axis about its centre of mass.
 """
 from __future__ import annotations
 
@@ -15,52 +24,91 @@ import numpy as np
 
 from .cell import cell_matrix
 
+_INVALID = "invalid or charged two-site String input"
 
-def parse_string_input(path) -> dict:
-    lines = [line.strip() for line in Path(path).read_text().splitlines()]
+# guest row columns: x y z eps sigma q m
+_G_XYZ, _G_EPS, _G_SIG, _G_Q, _G_MASS = slice(0, 3), 3, 4, 5, 6
+# framework row columns: ID sigma eps q m fa fb fc
+_F_ID, _F_SIG, _F_EPS, _F_Q, _F_FRAC = 0, 1, 2, 3, slice(5, 8)
+# condition columns: cutoff FH mass temperature
+_C_CUT, _C_MASS, _C_TEMP = 0, 2, 3
 
-    def after(prefix):
-        for index, line in enumerate(lines):
-            if line.startswith(prefix):
-                return index + 1
-        raise ValueError(f"input is missing the {prefix!r} header")
 
-    lengths = [float(v) for v in lines[after("La Lb Lc")].split()[:3]]
-    angles = [float(v) for v in lines[after("Alpha Beta Gamma")].split()[:3]]
-    cell = cell_matrix(lengths, angles)
-    conditions = [float(v) for v in lines[after("cutoff(A)")].split()]
-    g = after("Number of sites")
-    n_guest = int(lines[g])
-    guest = np.array([[float(v) for v in row.split()[:7]] for row in lines[g + 2:g + 2 + n_guest]])
-    f = after("Number of atoms")
-    n_frame = int(lines[f])
-    frame = np.array([[float(v) for v in row.split()[:8]] for row in lines[f + 2:f + 2 + n_frame]])
-    if (guest.shape != (2, 7) or frame.shape != (n_frame, 8) or len(conditions) < 4
-            or not np.isfinite(guest).all() or not np.isfinite(frame).all()
-            or not np.isfinite(conditions).all()
-            or not np.array_equal(frame[:, 0], np.arange(1, n_frame + 1))
-            or (guest[:, 5] != 0).any() or (frame[:, 3] != 0).any()
-            or (guest[:, 3:5] <= 0).any() or (guest[:, 6] <= 0).any()
-            or (frame[:, 1:3] <= 0).any() or conditions[0] <= 0 or conditions[3] <= 0):
-        raise ValueError("invalid or charged two-site String input")
-    mass = guest[:, 6]
-    if not np.isclose(mass.sum(), conditions[2], rtol=1e-7):
-        raise ValueError("guest site masses differ from declared total mass")
-    body = guest[:, :3]
-    separation = body[1] - body[0]
-    bond = np.linalg.norm(separation)
+class _Sections:
+    """The stripped lines of an input file, addressed by header prefix."""
+
+    def __init__(self, text: str):
+        self.lines = [line.strip() for line in text.splitlines()]
+
+    def value_line(self, header: str) -> int:
+        """Index of the line after the first line that starts with `header`."""
+        hit = next((i for i, line in enumerate(self.lines) if line.startswith(header)), None)
+        if hit is None:
+            raise ValueError(f"input is missing the {header!r} header")
+        return hit + 1
+
+    def floats(self, header: str, count: int | None = None) -> list[float]:
+        fields = self.lines[self.value_line(header)].split()
+        return [float(v) for v in (fields if count is None else fields[:count])]
+
+    def table(self, header: str, width: int) -> np.ndarray:
+        """The rows under a count line: count, one column-title line, then rows
+        (a file that ends early gives fewer rows)."""
+        at = self.value_line(header)
+        count = int(self.lines[at])
+        body = self.lines[at + 2:at + 2 + count]
+        rows = [[float(v) for v in row.split()[:width]] for row in body]
+        return np.array(rows, dtype=float).reshape(len(rows), -1) if rows else np.empty((0, width))
+
+
+def _well_formed(guest, frame, conditions, n_frame) -> bool:
+    if guest.shape != (2, 7) or frame.shape != (n_frame, 8) or len(conditions) < 4:
+        return False
+    if not (np.isfinite(guest).all() and np.isfinite(frame).all() and np.isfinite(conditions).all()):
+        return False
+    ids_in_order = np.array_equal(frame[:, _F_ID], np.arange(1, n_frame + 1))
+    neutral = not guest[:, _G_Q].any() and not frame[:, _F_Q].any()
+    positive = ((guest[:, [_G_EPS, _G_SIG, _G_MASS]] > 0).all()
+                and (frame[:, [_F_SIG, _F_EPS]] > 0).all()
+                and conditions[_C_CUT] > 0 and conditions[_C_TEMP] > 0)
+    return ids_in_order and neutral and positive
+
+
+def _canonical_body(xyz, mass, total_mass):
+    """Site x coordinates of the body laid on the x axis about its centre of mass,
+    and the bond length."""
+    bond_vector = xyz[1] - xyz[0]
+    bond = float(np.linalg.norm(bond_vector))
     if bond <= 0:
         raise ValueError("two distinct guest sites required")
-    director = separation / bond
-    com = (body * mass[:, None]).sum(0) / conditions[2]
-    canonical = np.zeros((2, 3))
-    canonical[:, 0] = (body - com) @ director
-    return {"cell": cell, "frame_frac": np.mod(frame[:, 5:8], 1),
-            "frame_sigma": frame[:, 1], "frame_epsilon": frame[:, 2],
-            "guest_xyz": canonical, "guest_sigma": guest[:, 4], "guest_epsilon": guest[:, 3],
-            "guest_mass": mass, "total_mass": float(conditions[2]),
-            "temperature": float(conditions[3]), "cutoff": float(conditions[0]),
-            "cell_lengths": lengths, "cell_angles_deg": angles, "bond_length": float(bond)}
+    com = (mass @ xyz) / total_mass
+    body = np.zeros((2, 3))
+    body[:, 0] = (xyz - com) @ (bond_vector / bond)
+    return body, bond
+
+
+def parse_string_input(path) -> dict:
+    sec = _Sections(Path(path).read_text())
+    lengths = sec.floats("La Lb Lc", 3)
+    angles = sec.floats("Alpha Beta Gamma", 3)
+    cell = cell_matrix(lengths, angles)
+    conditions = sec.floats("cutoff(A)")
+    n_frame = int(sec.lines[sec.value_line("Number of atoms")])
+    guest = sec.table("Number of sites", 7)
+    frame = sec.table("Number of atoms", 8)
+    if not _well_formed(guest, frame, conditions, n_frame):
+        raise ValueError(_INVALID)
+    mass = guest[:, _G_MASS]
+    total = conditions[_C_MASS]
+    if not np.isclose(mass.sum(), total, rtol=1e-7):
+        raise ValueError("guest site masses differ from declared total mass")
+    body, bond = _canonical_body(guest[:, _G_XYZ], mass, total)
+    return {"cell": cell, "frame_frac": np.mod(frame[:, _F_FRAC], 1),
+            "frame_sigma": frame[:, _F_SIG], "frame_epsilon": frame[:, _F_EPS],
+            "guest_xyz": body, "guest_sigma": guest[:, _G_SIG], "guest_epsilon": guest[:, _G_EPS],
+            "guest_mass": mass, "total_mass": float(total),
+            "temperature": float(conditions[_C_TEMP]), "cutoff": float(conditions[_C_CUT]),
+            "cell_lengths": lengths, "cell_angles_deg": angles, "bond_length": bond}
 
 
 def write_string_input(case: dict, path) -> None:

@@ -10,37 +10,63 @@ of the exact path and is not provided.
 """
 from __future__ import annotations
 
-import itertools
-
 import numpy as np
 
 from .orientation import regular_pose_grid
 
 
+def _finite(a) -> bool:
+    return bool(np.isfinite(a).all())
+
+
+def _positive_vector(values, length) -> bool:
+    a = np.asarray(values, dtype=float)
+    return a.shape == (length,) and _finite(a) and bool((a > 0).all())
+
+
 def validate_case(case):
-    """Reference physics.py:18-39."""
+    """Reject a case the exact path cannot evaluate, with the reference's
+    messages (physics.py:18-39): the framework (a 3x3 cell of positive
+    determinant and a nonempty (n, 3) finite fractional block), its positive LJ
+    parameters, a two-site guest with positive parameters and masses laid on the
+    body x axis, masses summing to the total, a positive cutoff and temperature."""
     cell = np.asarray(case["cell"], dtype=float)
     frame = np.asarray(case["frame_frac"], dtype=float)
-    if (cell.shape != (3, 3) or not np.isfinite(cell).all() or np.linalg.det(cell) <= 0
-            or frame.ndim != 2 or frame.shape[1] != 3 or not len(frame)
-            or not np.isfinite(frame).all()):
-        raise ValueError("invalid triclinic framework")
-    for key in ("frame_sigma", "frame_epsilon"):
-        values = np.asarray(case[key], dtype=float)
-        if values.shape != (len(frame),) or not np.isfinite(values).all() or (values <= 0).any():
-            raise ValueError("invalid framework LJ parameters")
-    for key in ("guest_sigma", "guest_epsilon", "guest_mass"):
-        values = np.asarray(case[key], dtype=float)
-        if values.shape != (2,) or not np.isfinite(values).all() or (values <= 0).any():
-            raise ValueError("invalid two-site guest")
+    framework_ok = (cell.shape == (3, 3) and frame.ndim == 2 and frame.shape[1] == 3
+                    and len(frame) > 0 and _finite(cell) and _finite(frame)
+                    and np.linalg.det(cell) > 0)
     body = np.asarray(case["guest_xyz"], dtype=float)
-    if body.shape != (2, 3) or not np.isfinite(body).all() or not np.allclose(body[:, 1:], 0):
-        raise ValueError("guest must have a two-site body x-axis")
-    if not np.isclose(sum(case["guest_mass"]), case["total_mass"], rtol=1e-6):
-        raise ValueError("guest mass mismatch")
-    if (not np.isfinite([case["cutoff"], case["temperature"]]).all()
-            or min(case["cutoff"], case["temperature"]) <= 0):
-        raise ValueError("invalid cutoff/temperature")
+    checks = (
+        (lambda: framework_ok, "invalid triclinic framework"),
+        (lambda: all(_positive_vector(case[k], len(frame)) for k in ("frame_sigma", "frame_epsilon")),
+         "invalid framework LJ parameters"),
+        (lambda: all(_positive_vector(case[k], 2)
+                     for k in ("guest_sigma", "guest_epsilon", "guest_mass")),
+         "invalid two-site guest"),
+        (lambda: body.shape == (2, 3) and _finite(body) and np.allclose(body[:, 1:], 0),
+         "guest must have a two-site body x-axis"),
+        (lambda: np.isclose(np.sum(case["guest_mass"]), case["total_mass"], rtol=1e-6),
+         "guest mass mismatch"),
+        (lambda: _finite([case["cutoff"], case["temperature"]])
+         and case["cutoff"] > 0 and case["temperature"] > 0,
+         "invalid cutoff/temperature"),
+    )
+    for holds, message in checks:
+        if not holds():
+            raise ValueError(message)
+
+
+def unit_directors(centers, axes, message):
+    """Validated (n, 3) FP64 centers and row-normalized (k, 3) directors."""
+    c = np.asarray(centers, dtype=float)
+    u = np.asarray(axes, dtype=float)
+    shapes_ok = c.ndim == 2 and c.shape[1:] == (3,) and u.ndim == 2 and u.shape[1:] == (3,)
+    if not (shapes_ok and np.isfinite(c).all() and np.isfinite(u).all()):
+        raise ValueError(message)
+    norm = np.linalg.norm(u, axis=1, keepdims=True)
+    if (norm <= 0).any():
+        raise ValueError(message)
+    return c, np.ascontiguousarray(u / norm)
 
 
 def body_offsets(case) -> np.ndarray:
@@ -49,6 +75,22 @@ def body_offsets(case) -> np.ndarray:
     mass = np.asarray(case["guest_mass"], dtype=float)
     center = (xyz * mass[:, None]).sum(0) / case["total_mass"]
     return xyz[:, 0] - center[0]
+
+
+def periodic_images(cell, frac, radius):
+    """Framework images reachable within `radius` of any point of the cell
+    (physics.py:49-56): with d_k = 1 / |column k of H^-1| the spacing of the
+    lattice planes k, whole-cell shifts n_k in [-r_k, r_k], r_k = ceil(radius /
+    d_k) + 1 (wrapped query and atom coordinates differ by less than one cell),
+    shift-major and atom-minor.  Returns (images [n_shifts * n_atoms][3], n_shifts)."""
+    cell = np.asarray(cell, dtype=float)
+    plane_spacing = 1.0 / np.linalg.norm(np.linalg.inv(cell), axis=0)
+    reach = np.ceil(radius / plane_spacing).astype(int) + 1
+    span = 2 * reach + 1
+    shifts = (np.indices(span).reshape(3, -1).T - reach).astype(float)
+    wrapped = np.mod(np.asarray(frac, dtype=float), 1)
+    images = (wrapped[None, :, :] + shifts[:, None, :]).reshape(-1, 3) @ cell
+    return images, len(shifts)
 
 
 class TriclinicSitePotential:
@@ -65,14 +107,9 @@ class TriclinicSitePotential:
         self.cell = np.asarray(case["cell"], dtype=np.float64)
         self.inverse = np.linalg.inv(self.cell)
         self.cutoff = float(case["cutoff"])
-        spacing = 1 / np.linalg.norm(self.inverse, axis=0)
-        # wrapped fractional query/atom components differ by less than one cell
-        reach = np.ceil(self.cutoff / spacing).astype(int) + 1
-        shifts = np.array(list(itertools.product(*[range(-r, r + 1) for r in reach])), dtype=float)
-        atoms = np.mod(case["frame_frac"], 1)
-        self.images = (atoms[None] + shifts[:, None]).reshape(-1, 3) @ self.cell
-        self.sigma = np.tile(case["frame_sigma"], len(shifts))
-        self.epsilon = np.tile(case["frame_epsilon"], len(shifts))
+        self.images, copies = periodic_images(self.cell, case["frame_frac"], self.cutoff)
+        self.sigma = np.tile(np.asarray(case["frame_sigma"], dtype=float), copies)
+        self.epsilon = np.tile(np.asarray(case["frame_epsilon"], dtype=float), copies)
         self._tree = None
 
     @property
@@ -103,26 +140,26 @@ class TriclinicSitePotential:
 
 
 def direct_pose_energy(case, centers, axes, potential=None):
-    """Energies [center][axis] of both guest sites (physics.py:93-106), GPU FP64."""
+    """Energies [center][axis] of both guest sites at fractional centers and
+    (normalized) directors (physics.py:93-106), FP64 on the GPU.  `potential`
+    is accepted for signature compatibility; the plan is built from `case`."""
     from .field import Plan
-    centers, axes = np.asarray(centers, dtype=float), np.asarray(axes, dtype=float)
-    if (centers.ndim != 2 or centers.shape[1] != 3 or axes.ndim != 2 or axes.shape[1] != 3
-            or not np.isfinite(centers).all() or not np.isfinite(axes).all()
-            or (np.linalg.norm(axes, axis=1) <= 0).any()):
-        raise ValueError("invalid center/director queries")
+    centers, axes = unit_directors(centers, axes, "invalid center/director queries")
     validate_case(case)
-    axes = axes / np.linalg.norm(axes, axis=1, keepdims=True)
     plan = Plan.from_case(case)
     return plan.field(plan.poses(centers, axes), precision="fp64")["energy"]
 
 
 def marginal_energy(energies, temperature):
-    """Boltzmann marginal over the last (orientation) axis (physics.py:126-130)."""
-    from scipy.special import logsumexp
+    """Boltzmann marginal over the last (orientation) axis (physics.py:126-130):
+    -T ln(mean_k exp(-E_k / T)), evaluated about the lowest energy of each row
+    so that no exponential overflows."""
     energy = np.asarray(energies, dtype=float)
-    if not np.isfinite(energy).all() or not np.isfinite(temperature) or temperature <= 0:
+    if not (_finite(energy) and np.isfinite(temperature) and temperature > 0):
         raise ValueError("finite angular energies and positive temperature required")
-    return -temperature * (logsumexp(-energy / temperature, axis=-1) - np.log(energy.shape[-1]))
+    low = energy.min(axis=-1, keepdims=True)
+    mean_weight = np.exp((low - energy) / temperature).mean(axis=-1)
+    return low[..., 0] - temperature * np.log(mean_weight)
 
 
 def uniform_fields(case, size=60, orientations=64, energy_mode="direct", precision="fp64"):

@@ -1,21 +1,31 @@
 """Exact triclinic orientation-resolved Vext field generation (``string-vext``).
 
-Mirrors ``string_vext.vext`` (reference src/string_vext/vext.py:21-80): the
-same NPZ channels, dtypes and layouts --
-  orientation_K [K][n][n][n] float32, marginal_K [n][n][n] float32,
-  site_K [2][n][n][n] float32, axes [K][3] float32, grid = asinh(marginal/1000),
-  cell_matrix, atom_fractional, framework_sigma_A, framework_epsilon_K (FP64) --
-the same CLI flags, exclusive output creation and provenance JSON.  With the
-default kernel (libsv_vext.so) every channel is produced on the GPU in one
-fused evaluation; ``--precision mixed`` selects the FP32-pair / FP64-accumulate
-kernel.
+Contract of ``string_vext.vext`` (reference src/string_vext/vext.py:21-80):
+``calculate_vext(case, size, orientations, kernel)`` returns the NPZ channels
+
+  orientation_K [K][n][n][n] float32   pose energies, orientation-major
+  marginal_K    [n][n][n]    float32   -T ln(mean_k exp(-E_k / T))
+  site_K        [2][n][n][n] float32   single-site field, once per site
+  axes          [K][3]       float32   Fibonacci directors
+  grid          [n][n][n]    float32   asinh(marginal / 1000)
+  cell_matrix, atom_fractional, framework_sigma_A, framework_epsilon_K  (FP64)
+
+and ``main`` is the CLI (--input --output --size --orientations --kernel),
+which refuses to overwrite, publishes the NPZ atomically under its final
+name, and writes a provenance JSON beside it.  With the default kernel
+(libsv_vext.so) every channel comes from one fused GPU evaluation; with a
+library that only exports ``evaluate_reused_neighbors`` the energies go
+through that C ABI in 512-center blocks.  Added here: ``--precision mixed``
+(FP32 pair arithmetic, FP64 accumulation) and ``--near-energy``.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import hashlib
 import json
 import os
+import tempfile
 from pathlib import Path
 
 import numpy as np
@@ -24,95 +34,124 @@ from .input import parse_string_input
 from .orientation import regular_pose_grid
 
 DEFAULT_KERNEL = Path(__file__).resolve().parent / "libsv_vext.so"
+ENERGY_MODE = "exact_triclinic_LJ_neighbor_reuse"
 
 
-def _has_batched(kernel: Path) -> bool:
-    import ctypes
+def _exports_plans(kernel: Path) -> bool:
     return hasattr(ctypes.CDLL(str(Path(kernel).resolve())), "sv_plan_create")
+
+
+def _gpu_channels(case, size, axes, precision, near_energy_K):
+    """orientation [K][n^3], marginal, grid, site [n^3] from one fused launch."""
+    from .field import Plan
+    plan = Plan.from_case(case, near_energy_K=near_energy_K)
+    out = plan.field(plan.grid(size, axes), precision=precision, energy=False, orientation=True,
+                     marginal=True, grid=True, site=True, temperature=case["temperature"])
+    return out["orientation"], out["marginal"], out["grid"], out["site"]
+
+
+def _abi_channels(case, centers, axes, kernel):
+    """The same channels with the energies from the drop-in C ABI."""
+    from .native import NativeReusePotential
+    from .physics import TriclinicSitePotential, marginal_energy
+    energy = NativeReusePotential(case, kernel).evaluate(centers, axes)
+    marginal = marginal_energy(energy, case["temperature"]).astype(np.float32)
+    site = TriclinicSitePotential(case).evaluate(centers @ case["cell"], case["guest_sigma"][0],
+                                                 case["guest_epsilon"][0])
+    return (energy.T.astype(np.float32), marginal, np.arcsinh(marginal / 1000).astype(np.float32),
+            site.astype(np.float32))
 
 
 def calculate_vext(case: dict, size: int = 60, orientations: int = 64,
                    kernel: Path = DEFAULT_KERNEL, precision: str = "fp64",
                    near_energy_K: float | None = None) -> dict[str, np.ndarray]:
-    from .field import DEFAULT_NEAR_ENERGY_K, Plan
-    from .physics import TriclinicSitePotential, marginal_energy, validate_case
-    centers, axes = regular_pose_grid(size, orientations)
+    from .field import DEFAULT_NEAR_ENERGY_K
+    from .physics import validate_case
     validate_case(case)
-    near = DEFAULT_NEAR_ENERGY_K if near_energy_K is None else near_energy_K
-    if _has_batched(kernel):
-        plan = Plan.from_case(case, near_energy_K=near)
-        out = plan.field(plan.grid(size, axes), precision=precision, energy=False,
-                         orientation=True, marginal=True, grid=True, site=True,
-                         temperature=case["temperature"])
-        orientation = out["orientation"].reshape(orientations, size, size, size)
-        marginal = out["marginal"].reshape(size, size, size)
-        site = out["site"].reshape(size, size, size)
-        grid = out["grid"].reshape(size, size, size)
+    centers, axes = regular_pose_grid(size, orientations)
+    if _exports_plans(kernel):
+        near = DEFAULT_NEAR_ENERGY_K if near_energy_K is None else near_energy_K
+        orientation, marginal, grid, site = _gpu_channels(case, size, axes, precision, near)
     else:
-        from .native import NativeReusePotential
-        energy = NativeReusePotential(case, kernel).evaluate(centers, axes)
-        orientation = energy.T.reshape(orientations, size, size, size).astype(np.float32)
-        marginal = marginal_energy(energy, case["temperature"]).reshape(size, size, size)
-        marginal = marginal.astype(np.float32)
-        site = TriclinicSitePotential(case).evaluate(
-            centers @ case["cell"], case["guest_sigma"][0], case["guest_epsilon"][0])
-        site = site.reshape(size, size, size).astype(np.float32)
-        grid = np.arcsinh(marginal / 1000).astype(np.float32)
-    result = {
-        "orientation_K": orientation,
-        "marginal_K": marginal,
-        "site_K": np.stack((site, site)).astype(np.float32),
+        orientation, marginal, grid, site = _abi_channels(case, centers, axes, kernel)
+    volume = (size, size, size)
+    fields = {
+        "orientation_K": orientation.reshape(orientations, *volume),
+        "marginal_K": marginal.reshape(volume),
+        "site_K": np.broadcast_to(site.reshape(volume), (2, *volume)).astype(np.float32),
         "axes": axes.astype(np.float32),
-        "cell_matrix": np.asarray(case["cell"], dtype=np.float64),
-        "atom_fractional": np.asarray(case["frame_frac"], dtype=np.float64),
-        "framework_sigma_A": np.asarray(case["frame_sigma"], dtype=np.float64),
-        "framework_epsilon_K": np.asarray(case["frame_epsilon"], dtype=np.float64),
-        "grid": grid,
+        "grid": grid.reshape(volume),
     }
-    if any(not np.isfinite(value).all() for value in result.values()):
+    for name, key in (("cell_matrix", "cell"), ("atom_fractional", "frame_frac"),
+                      ("framework_sigma_A", "frame_sigma"), ("framework_epsilon_K", "frame_epsilon")):
+        fields[name] = np.asarray(case[key], dtype=np.float64)
+    if not all(np.isfinite(v).all() for v in fields.values()):
         raise ValueError("nonfinite Vext output")
-    return result
+    return fields
+
+
+def _sha256(path: Path) -> str:
+    digest = hashlib.sha256()
+    with open(path, "rb") as handle:
+        for block in iter(lambda: handle.read(1 << 20), b""):
+            digest.update(block)
+    return digest.hexdigest()
+
+
+def _publish_npz(destination: Path, fields: dict) -> None:
+    """Write the NPZ under a unique temporary name in the destination's folder
+    and hard-link it to its final name: readers never see a partial file, and
+    an existing destination makes the link (and the call) fail."""
+    destination.parent.mkdir(parents=True, exist_ok=True)
+    fd, staging = tempfile.mkstemp(prefix=destination.name + ".", suffix=".partial",
+                                   dir=destination.parent)
+    try:
+        with os.fdopen(fd, "wb") as handle:
+            np.savez_compressed(handle, **fields)
+        os.link(staging, destination)
+    finally:
+        os.unlink(staging)
+
+
+def _provenance(args, case) -> dict:
+    sources = {"input": args.input, "output": args.output, "kernel": args.kernel}
+    record = {"input": str(args.input.resolve())}
+    record.update({f"{name}_sha256": _sha256(path) for name, path in sources.items()})
+    record.update(size=args.size, orientations=args.orientations, energy_mode=ENERGY_MODE,
+                  precision=args.precision, cell_lengths_A=case["cell_lengths"],
+                  cell_angles_deg=case["cell_angles_deg"],
+                  guest_sigma_A=np.asarray(case["guest_sigma"]).tolist(),
+                  guest_epsilon_K=np.asarray(case["guest_epsilon"]).tolist())
+    return record
+
+
+def _arguments(argv):
+    cli = argparse.ArgumentParser(prog="string-vext", description=__doc__.splitlines()[0])
+    cli.add_argument("--input", type=Path, required=True,
+                     help="String input.dat (framework, guest, cell, cutoff, temperature)")
+    cli.add_argument("--output", type=Path, required=True,
+                     help="NPZ to create; it and its .json record must not exist yet")
+    cli.add_argument("--size", type=int, default=60, help="centers per lattice direction")
+    cli.add_argument("--orientations", type=int, default=64, help="Fibonacci directors")
+    cli.add_argument("--kernel", type=Path, default=DEFAULT_KERNEL,
+                     help="shared library exporting evaluate_reused_neighbors")
+    cli.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    cli.add_argument("--near-energy", type=float, default=None,
+                     help="mixed precision: pair terms that can exceed this (K) run in FP64")
+    return cli.parse_args(argv)
 
 
 def main(argv=None):
-    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    parser.add_argument("--input", type=Path, required=True, help="native String input.dat")
-    parser.add_argument("--output", type=Path, required=True, help="new NPZ destination")
-    parser.add_argument("--size", type=int, default=60, help="grid points per axis")
-    parser.add_argument("--orientations", type=int, default=64)
-    parser.add_argument("--kernel", type=Path, default=DEFAULT_KERNEL)
-    parser.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
-    parser.add_argument("--near-energy", type=float, default=None,
-                        help="mixed precision: pair terms that can exceed this (K) run in FP64")
-    args = parser.parse_args(argv)
+    args = _arguments(argv)
     record_path = args.output.with_suffix(".json")
-    if args.output.exists() or record_path.exists():
+    if any(p.exists() for p in (args.output, record_path)):
         raise FileExistsError("Vext output or provenance record already exists")
     case = parse_string_input(args.input)
-    fields = calculate_vext(case, args.size, args.orientations, args.kernel, args.precision,
-                            args.near_energy)
-    args.output.parent.mkdir(parents=True, exist_ok=True)
-    temporary = args.output.with_suffix(".npz.tmp")
-    try:
-        with temporary.open("xb") as handle:
-            np.savez_compressed(handle, **fields)
-        os.link(temporary, args.output)
-    finally:
-        temporary.unlink(missing_ok=True)
-    record = {"input": str(args.input.resolve()),
-              "input_sha256": hashlib.sha256(args.input.read_bytes()).hexdigest(),
-              "output_sha256": hashlib.sha256(args.output.read_bytes()).hexdigest(),
-              "kernel_sha256": hashlib.sha256(args.kernel.read_bytes()).hexdigest(),
-              "size": args.size, "orientations": args.orientations,
-              "energy_mode": "exact_triclinic_LJ_neighbor_reuse",
-              "precision": args.precision,
-              "cell_lengths_A": case["cell_lengths"],
-              "cell_angles_deg": case["cell_angles_deg"],
-              "guest_sigma_A": case["guest_sigma"].tolist(),
-              "guest_epsilon_K": case["guest_epsilon"].tolist()}
+    _publish_npz(args.output, calculate_vext(case, args.size, args.orientations, args.kernel,
+                                             args.precision, args.near_energy))
+    record = _provenance(args, case)
     with record_path.open("x") as handle:
-        json.dump(record, handle, indent=2)
-        handle.write("\n")
+        handle.write(json.dumps(record, indent=2) + "\n")
     print(json.dumps(record, sort_keys=True))
 
 

@@ -1,0 +1,187 @@
+"""GPU parity at the benchmark workloads' own sizes, checked the way the
+reference audits its full grids (tests/cof/reference_project/graph_vext/
+exact_neighbor_reuse_fullgrid.py:55-80: accessible-pose error of the mixed
+field against FP64 on every pose of the 60^3 x 64 grid).
+
+* the full-size single field: 60^3 grid x 64 orientations over the seeded
+  1200-atom framework (37 x 37 x 14 A, gamma 120, 4 LJ types), mixed against
+  FP64 on all 13.8 M poses at two tile sizes, and a contiguous slab of 1,024
+  grid centers plus every center with an accessible orientation, x 64
+  orientations, against the FP64 oracle, pair census included;
+* differentiable poses: one full-size framework x 65,536 uniform random poses
+  through torch.autograd, mixed against FP64 at full size and a 2,000-pose
+  subset against the oracle's analytic gradient;
+* batched strings: 64 strings x 96 images over 8 full-size and 8 porous
+  frameworks, HIP-graph replay against eager for 50 iterations, and two
+  iterations of one 96-image string against the CPU restatement.
+Tolerances: FP64 form <= 1e-9 relative (1 K floor; the required bound is
+1e-6); mixed form <= 1e-3 K on accessible poses (E < 2000 K, |E| < 1e4 K) and
+<= 1e-5 relative with a 100 K floor on every pose.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZE, K = 60, 64
+
+
+@pytest.fixture(scope="module")
+def full_case():
+    from string_vext_amd import synthetic
+    return synthetic.full_field_case()
+
+
+@pytest.fixture(scope="module")
+def full_plan(full_case):
+    from string_vext_amd.field import Plan
+    return Plan.from_case(full_case)
+
+
+def _full_field(plan, tile_centers):
+    from string_vext_amd.orientation import regular_pose_grid
+    _, axes = regular_pose_grid(SIZE, K)
+    poses = plan.grid(SIZE, axes, tile_centers=tile_centers)
+    return (plan.field(poses, precision="fp64")["energy"],
+            plan.field(poses, precision="mixed")["energy"])
+
+
+@pytest.mark.parametrize("tile_centers", [64, 256])
+def test_full_field_mixed_against_fp64_on_every_pose(oracle, full_plan, tile_centers):
+    e64, emx = _full_field(full_plan, tile_centers)
+    assert e64.shape == (SIZE ** 3, K) and np.isfinite(e64).all()
+    m = oracle.error_metrics(e64, emx)
+    print(f"tile {tile_centers}: {m}", flush=True)
+    assert m["poses"] == SIZE ** 3 * K
+    assert m["accessible_poses"] > 0
+    assert m["accessible_max_abs_error_K"] < 1e-3
+    assert m["all_max_relative_error_floor100K"] < 1e-5
+
+
+def test_full_field_slab_against_oracle_with_census(oracle, full_case, full_plan):
+    from string_vext_amd.orientation import regular_pose_grid
+    centers, axes = regular_pose_grid(SIZE, K)
+    # a contiguous slab (ia = 0, ib = 0..17) plus every grid center with an
+    # accessible orientation on the GPU's FP64 field (the slab itself has none)
+    e64_full, _ = _full_field(full_plan, 64)
+    open_centers = np.nonzero((e64_full < 2000.0).any(axis=1))[0]
+    assert len(open_centers) > 0
+    slab = np.concatenate([centers[:1024], centers[open_centers]])
+    setup = oracle.ReuseSetup(full_case)
+    ref = oracle.reuse_energy(full_case, slab, axes, setup=setup)
+    poses = full_plan.poses(slab, axes)
+    e64 = full_plan.field(poses, precision="fp64")["energy"]
+    emx = full_plan.field(poses, precision="mixed")["energy"]
+    rel64 = np.abs(e64 - ref) / np.maximum(np.abs(ref), 1.0)
+    m = oracle.error_metrics(ref, emx)
+    print(f"slab fp64 max rel {rel64.max():.3e}; mixed {m}", flush=True)
+    assert rel64.max() < 1e-9
+    assert m["all_max_relative_error_floor100K"] < 1e-5
+    assert m["accessible_poses"] > 0
+    assert m["accessible_max_abs_error_K"] < 1e-3
+    pairs = full_plan.census(poses)
+    assert pairs == oracle.count_pairs(full_case, slab, axes, setup=setup)
+    # about 1,270 in-cutoff pairs per pose at this density
+    assert 1000 * len(slab) * K < pairs < 1500 * len(slab) * K
+
+
+@pytest.fixture(scope="module")
+def random_poses(full_case):
+    rng = np.random.default_rng(20261105)
+    frac = rng.random((65536, 3))
+    u = rng.normal(size=(65536, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    return frac @ np.asarray(full_case["cell"]), u
+
+
+def _autograd(plan, centers, axes, precision):
+    import torch
+    from string_vext_amd.autograd import check_poses, pose_energy
+    c = torch.tensor(centers, device="cuda", requires_grad=True)
+    u = torch.tensor(axes, device="cuda", requires_grad=True)
+    e = pose_energy(plan, c, u, precision)
+    e.sum().backward()
+    check_poses(plan)
+    return e.detach().cpu().numpy(), c.grad.cpu().numpy(), u.grad.cpu().numpy()
+
+
+def _grad_rel(g, ref):
+    scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)
+    return float((np.abs(g - ref) / scale).max())
+
+
+def test_full_size_poses_forward_backward(oracle, full_case, full_plan, random_poses):
+    centers, axes = random_poses
+    e64, gc64, gu64 = _autograd(full_plan, centers, axes, "fp64")
+    emx, gcmx, gumx = _autograd(full_plan, centers, axes, "mixed")
+    m = oracle.error_metrics(e64, emx)
+    print(f"65,536 poses mixed vs fp64: {m}; grad rel {_grad_rel(gcmx, gc64):.2e} "
+          f"{_grad_rel(gumx, gu64):.2e}", flush=True)
+    assert m["accessible_poses"] > 0
+    assert m["accessible_max_abs_error_K"] < 1e-3
+    assert m["all_max_relative_error_floor100K"] < 1e-5
+    assert _grad_rel(gcmx, gc64) < 1e-4 and _grad_rel(gumx, gu64) < 1e-4
+    sub = np.linspace(0, len(centers) - 1, 2000).round().astype(int)
+    e_ref, gc_ref, gu_ref = oracle.pose_grad(full_case, centers[sub], axes[sub])
+    rel = np.abs(e64[sub] - e_ref) / np.maximum(np.abs(e_ref), 1.0)
+    assert rel.max() < 1e-9
+    assert _grad_rel(gc64[sub], gc_ref) < 1e-9 and _grad_rel(gu64[sub], gu_ref) < 1e-9
+    m = oracle.error_metrics(e_ref, emx[sub])
+    assert m["all_max_relative_error_floor100K"] < 1e-5
+    assert _grad_rel(gcmx[sub], gc_ref) < 1e-4 and _grad_rel(gumx[sub], gu_ref) < 1e-4
+
+
+@pytest.fixture(scope="module")
+def string_batch():
+    """8 full-size and 8 porous (400-atom) frameworks x 4 strings each = 64
+    strings x 96 images, straight fractional paths along a, b, c."""
+    from string_vext_amd import synthetic
+    from string_vext_amd.field import Plan
+    from string_vext_amd.strings import straight_strings
+    plans, cases, init, dirs = [], [], [], []
+    for n_atoms in (1200, 400):
+        for k in range(8):
+            case = synthetic.full_field_case(seed=synthetic.FULL_SEED + k, n_atoms=n_atoms)
+            plan = Plan.from_case(case)
+            for j in range(4):
+                d = 1 + (k * 4 + j) % 3
+                plans.append(plan)
+                cases.append(case)
+                dirs.append(d)
+                init.append(straight_strings(plan, d, 1, 96, seed=1000 * k + j, scan=8,
+                                             orientations=4, probes=12)[0])
+    return plans, cases, np.stack(init), np.array(dirs)
+
+
+def test_string_batch_graph_replay_equals_eager(string_batch):
+    from string_vext_amd.strings import StringSystem
+    plans, _, x0, dirs = string_batch
+    assert x0.shape == (64, 96, 6)
+    for precision in ("fp64", "mixed"):
+        eager = StringSystem(plans, x0, dirs, precision=precision)
+        graph = StringSystem(plans, x0, dirs, precision=precision)
+        for _ in range(2):
+            eager.run(25, graph=False)
+            graph.run(25, graph=True)
+            for u, v in zip(eager.state(), graph.state()):
+                assert np.array_equal(u, v)
+        assert (eager.state()[2] == 50).all()
+
+
+def test_string_iterations_match_cpu_restatement_at_96_images(oracle, string_batch):
+    from oracle import string_oracle as SO
+    from string_vext_amd.strings import StringSystem
+    plans, cases, x0, dirs = string_batch
+    for s in (0, 40):                              # one full-size, one porous framework
+        case = cases[s]
+        setup = oracle.ReuseSetup(case)
+        system = StringSystem([plans[s]], x0[s:s + 1], dirs[s:s + 1], precision="fp64")
+        ref = x0[s]
+        for it in range(2):
+            ref = SO.iterate(case, ref, int(dirs[s]), it, setup=setup)[0]
+        system.run(2, graph=True)
+        path, energy, _, _ = system.state()
+        np.testing.assert_allclose(path[0, :, :3], ref[:, :3], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(path[0, :, 3:], ref[:, 3:], rtol=0, atol=1e-9)
+        e_ref = SO.energy_grad(case, ref, setup)[0]
+        np.testing.assert_allclose(energy[0], e_ref, rtol=1e-10, atol=1e-7)

@@ -122,3 +122,18 @@ def test_cli_refuses_existing_output(tmp_path):
     out.write_bytes(b"")
     with pytest.raises(FileExistsError):
         main(["--input", str(tmp_path / "missing.dat"), "--output", str(out)])
+
+
+def test_candidate_csr_equals_ball_queries(oracle):
+    """The drop-in route's CSR (sparse distance matrix) holds, for every center,
+    the same images as the oracle's cKDTree ball queries (native.py:47-55)."""
+    from string_vext_amd.native import candidate_csr
+    case, centers, _, _ = _golden.load("c2h4")
+    setup = oracle.ReuseSetup(case)
+    cart = np.mod(centers, 1) @ setup.cell
+    starts, indices = candidate_csr(setup.tree, cart, setup.radius)
+    ref_starts, ref_indices = oracle.csr(setup.tree, cart, setup.radius)
+    assert starts.dtype == np.int64 and indices.dtype == np.int32
+    np.testing.assert_array_equal(starts, ref_starts)
+    for c in range(len(cart)):
+        assert set(indices[starts[c]:starts[c + 1]]) == set(ref_indices[ref_starts[c]:ref_starts[c + 1]])

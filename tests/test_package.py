@@ -1,0 +1,51 @@
+"""The boundary ships as a package whose ``string-vext`` console script (the
+reference's script name, its pyproject.toml [project.scripts]) resolves to this
+package's CLI, which keeps the reference's refusal to overwrite an output."""
+import importlib
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _scripts():
+    import tomli
+    with open(ROOT / "pyproject.toml", "rb") as handle:
+        return tomli.load(handle)["project"]["scripts"]
+
+
+def test_console_script_resolves_to_the_cli():
+    target = _scripts()["string-vext"]
+    module, _, attr = target.partition(":")
+    main = getattr(importlib.import_module(module), attr)
+    from string_vext_amd.vext import main as cli
+    assert main is cli
+
+
+def test_console_script_refuses_an_existing_output(tmp_path):
+    module, _, attr = _scripts()["string-vext"].partition(":")
+    main = getattr(importlib.import_module(module), attr)
+    out = tmp_path / "vext.npz"
+    out.write_bytes(b"kept")
+    with pytest.raises(FileExistsError, match="already exists"):
+        main(["--input", str(tmp_path / "input.dat"), "--output", str(out)])
+    assert out.read_bytes() == b"kept"
+    record = tmp_path / "other.json"
+    record.write_text("{}")
+    with pytest.raises(FileExistsError):
+        main(["--input", str(tmp_path / "input.dat"), "--output", str(tmp_path / "other.npz")])
+
+
+def test_npz_is_published_whole_and_never_overwritten(tmp_path):
+    import numpy as np
+    from string_vext_amd.vext import _publish_npz
+    out = tmp_path / "sub" / "vext.npz"
+    _publish_npz(out, {"a": np.arange(3.0)})
+    with np.load(out) as z:
+        assert z["a"].tolist() == [0.0, 1.0, 2.0]
+    with pytest.raises(FileExistsError):
+        _publish_npz(out, {"a": np.zeros(2)})
+    with np.load(out) as z:
+        assert z["a"].tolist() == [0.0, 1.0, 2.0]
+    assert sorted(p.name for p in out.parent.iterdir()) == ["vext.npz"]   # no staging files left
