@@ -81,10 +81,12 @@ int sv_plan_info(void *plan, int *n_types, double *keep_radius);
 
 /* ------------------------------------------------------------------------ */
 /* Pose sets: every (center, axis) pair of n_centers fractional centers and  */
-/* n_axes unit directors (the product evaluated by NativeReusePotential.     */
+/* n_axes directors (the product evaluated by NativeReusePotential.          */
 /* evaluate, native.py:26-68).  Centers are wrapped into [0,1) like          */
-/* np.mod(centers, 1) (native.py:35).  sv_poses_grid uses the regular pose   */
-/* grid centers (i+0.5)/size on an ij-indexed meshgrid (orientation.py:7-16).*/
+/* np.mod(centers, 1) (native.py:35) and directors are normalized to u/|u|   */
+/* like native.py:34 (finite nonzero directors required, else code 1).       */
+/* sv_poses_grid uses the regular pose grid centers (i+0.5)/size on an       */
+/* ij-indexed meshgrid (orientation.py:7-16).                                */
 /* ------------------------------------------------------------------------ */
 int sv_poses_create(void *plan, int n_centers, const double *centers_frac, int n_axes,
                     const double *axes, void **poses);
@@ -147,7 +149,10 @@ int sv_batch_set_persistent(void *batch, int blocks_per_cu);
 /* counterpart in the reference, whose String forces are forward differences */
 /* of the energy (native/string/source_code/global.h:1281-1324).             */
 /*   centers_cart [n][3] Cartesian (wrapped internally), axes [n][3] used as  */
-/*   given (callers normalize).                                              */
+/*   given (callers normalize): the sites are c + o_s u in both precisions.  */
+/* Each pose is evaluated as its two site positions, E = phi(p_0) + phi(p_1),*/
+/* dE/dc = g_0 + g_1, dE/du = o_0 g_0 + o_1 g_1 with g_s = grad phi(p_s).    */
+/* A pose list is destroyed with sv_poses_destroy.                           */
 /* sv_pose_list_eval writes DEVICE arrays (each may be NULL):                */
 /*   energy [n] FP64; grad_center [n][3] = g_i dE_i/dc_i;                     */
 /*   grad_axis [n][3] = g_i dE_i/du_i, with g = grad_out [n] (device) or 1.    */
@@ -163,8 +168,10 @@ int sv_pose_list_eval(void *poses, int precision, const double *grad_out, double
 /* Device-resident independent poses (no counterpart in the reference; for   */
 /* callers whose poses already live on the GPU, e.g. torch.autograd).        */
 /* sv_pose_grid_create fixes a grid of spatial tiles over the cell, sized    */
-/* for about n_expected poses at tile_centers per tile (0 = 64), and builds  */
-/* every tile's candidate list once.  sv_pose_grid_eval takes DEVICE arrays  */
+/* for the 2 n_expected site positions of about n_expected poses at           */
+/* tile_centers sites per tile (0: the bin counts that minimize the expected */
+/* candidate work for uniformly spread poses), and builds every tile's       */
+/* candidate list once.  sv_pose_grid_eval takes DEVICE arrays               */
 /* centers_cart [n][3] and axes [n][3] (FP64), bins the poses into the tiles */
 /* on the GPU and evaluates them exactly like sv_pose_list_eval (same        */
 /* outputs, all DEVICE pointers, each may be NULL).  Any n >= 1 is accepted  */
@@ -197,6 +204,22 @@ int sv_pose_grid_check(void *grid);
 /* grad_out are this call's.  Returns 1 when no poses have been binned.       */
 int sv_pose_grid_reeval_async(void *grid, void *stream, int precision, const double *grad_out,
                               double *energy, double *grad_center, double *grad_axis);
+/* Exact number of in-cutoff (site, image) pairs (FP64 test) of the poses of  */
+/* the grid's last evaluation: the pair-eval count of the throughput metric. */
+/* Synchronous; returns 1 when no poses have been binned.                    */
+int sv_pose_grid_census(void *grid, unsigned long long *pairs);
+/* With timing on, HIP events on the evaluation's stream bracket its phases; */
+/* sv_pose_grid_timings waits for the last timed evaluation and returns its  */
+/* milliseconds: [0] binning (0 for a re-evaluation), [1] site kernel,        */
+/* [2] pose sums.                                                             */
+int sv_pose_grid_set_timing(void *grid, int on);
+/* Blocks that share each run of up to 64 sites (1..8; default 1): each      */
+/* takes every parts-th batch of the run's candidate list and writes its own */
+/* partial sums, which the pose sums add in part order.  More parts spread   */
+/* the work of a call over more blocks (a shorter last wave of blocks) at a  */
+/* fixed cost per block.                                                      */
+int sv_pose_grid_set_parts(void *grid, int parts);
+int sv_pose_grid_timings(void *grid, double *ms3);
 
 /* ------------------------------------------------------------------------ */
 /* Batched String method: n_strings independent strings of n_images images. */

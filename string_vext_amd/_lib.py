@@ -50,6 +50,10 @@ SIGNATURES = [
     ("sv_pose_grid_eval_async", _I, [_P, _P, _I, _P, _P, _I, _P, _P, _P, _P]),
     ("sv_pose_grid_check", _I, [_P]),
     ("sv_pose_grid_reeval_async", _I, [_P, _P, _I, _P, _P, _P, _P]),
+    ("sv_pose_grid_census", _I, [_P, _P]),
+    ("sv_pose_grid_set_timing", _I, [_P, _I]),
+    ("sv_pose_grid_set_parts", _I, [_P, _I]),
+    ("sv_pose_grid_timings", _I, [_P, _P]),
     ("sv_strings_create", _I, [_I, _P, _I, _P, _P, _D, _D, _I, _P]),
     ("sv_strings_run", _I, [_P, _I, _I]),
     ("sv_strings_state", _I, [_P, _P, _P, _P, _P]),

@@ -11,7 +11,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
-SOURCES = ["sv_kernels.hip", "sv_runtime.hip", "sv_string.hip"]
+SOURCES = ["sv_kernels.hip", "sv_sites.hip", "sv_runtime.hip", "sv_string.hip"]
 LIB = HERE / "libsv_vext.so"
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.

@@ -97,13 +97,15 @@ struct Job {
   unsigned long long* pair_count;  // census output (pairs with d^2 < rc^2)
   int* bad;                  // set to 1 when a nonfinite energy is produced
   int pad2;
-  // ---- independent poses (one director per center) ------------------------
-  const double* pose_axes64; // [n_centers][3] director of each sorted slot
-  const float* pose_axes32;
-  const double* grad_out;    // [n_centers] upstream gradient by caller index (null = 1)
-  double* pose_e;            // [n_centers] FP64 energy by caller index
-  double* grad_c;            // [n_centers][3] dE/d(center), Cartesian
-  double* grad_u;            // [n_centers][3] dE/d(director), director used as given
+  // ---- candidate list layout ------------------------------------------------
+  int n_cls;                 // classes per (tile, type) segment: 1, or 2 for site grids,
+                             // whose class 0 holds the images within sqrt(split2[type])
+                             // of the tile centre and class 1 the rest
+  int pad3;
+  double list_scale;         // list32 xyz = tile-relative Cartesian * list_scale
+  const double* split2;      // [n_types] class-0 radius^2 (n_cls == 2)
+  // ---- site decomposition of independent poses -----------------------------
+  const float* nsite2f;      // [n_types] (site near-contact radius / rc)^2
 };
 
 // Global tile table entry (one per tile of every job).
@@ -120,6 +122,17 @@ struct Tile {
 // A 3x3 matrix passed by value to kernels (row-major).
 struct Mat3 {
   double m[9];
+};
+
+// Work unit of the site form of independent poses: up to 64 sites of one bin
+// in slots [first, first + count).  whole: the run is made of whole cells of
+// the bin's 4 x 4 x 4 sub-grid, so its set of sites is fixed by the cell
+// counts; 0 when it is part of a single cell holding more than 64 sites.
+struct SiteRun {
+  int bin;
+  int first;
+  int count;
+  int whole;
 };
 
 // Site-field work chunk: up to 64 member centers of one tile.
@@ -153,27 +166,56 @@ hipError_t launch_census(const Job* d_jobs, int n_jobs, const long long* d_job_w
                          hipStream_t s);
 hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job_first_center,
                            long long n_centers_total, hipStream_t s);
-// n_chunks bounds the grid; d_n_chunks (device, may be null) holds the count.
-hipError_t launch_pose(int precision, bool grad, const Job* d_jobs, const SiteChunk* d_chunks,
-                       int n_chunks, const int* d_n_chunks, const long long* d_offsets,
-                       const float4* d_list32, const double* d_list64, hipStream_t s);
-hipError_t launch_pose_bin(int n, const double* d_centers, const double* d_axes, const Mat3& Hinv,
-                           const int m[3], int* d_bin, int* d_rank, int* d_counts, int* d_bad,
-                           hipStream_t s);
-hipError_t launch_pose_tile_scan(const int* d_counts, int n_bins, int job, int* d_first,
-                                 SiteChunk* d_chunks, int* d_n_chunks, hipStream_t s);
-hipError_t launch_pose_scatter(int n, const double* d_centers, const double* d_axes, const Mat3& H,
-                               const Mat3& Hinv, const int m[3], const int* d_bin,
-                               const int* d_rank, const int* d_first, double* d_center_rel,
-                               int* d_center_id, int* d_tile_of, double* d_axes64, float* d_axes32,
-                               hipStream_t s);
 hipError_t launch_zero(int* p, int n, hipStream_t s);
-// Sets the per-call pose pointers of job 0 from kernel arguments (stream-ordered).
-// Patches the pose job's arrays and clears zero[0..n_zero) in one launch.
-hipError_t launch_job_patch(Job* d_job, const double* center_rel, const int* center_id,
-                            const int* tile_of, const double* axes64, const float* axes32, int n,
-                            const double* grad_out, double* pose_e, double* grad_c,
-                            double* grad_u, int* zero, int n_zero, hipStream_t s);
+// ---- independent poses evaluated site by site (sv_sites.hip) -----------------
+// Site binning: every site goes to cell 64 b + sub, where b is its bin of
+// the m[0] x m[1] x m[2] grid and sub the Morton index of its cell of the
+// bin's 4 x 4 x 4 sub-grid, with the lattice axes order & 3, (order >> 2) & 3,
+// (order >> 4) & 3 (longest bin edge first) as the most significant.
+hipError_t launch_site_bin(long long n_sites, const double* centers, const double* axes,
+                           double o0, double o1, const Mat3& Hinv, const int m[3], int order,
+                           int* cell, int* rank, int* cell_counts, int* bad, hipStream_t s);
+// First slot of every cell (scan of the counts) and the run table: the cells
+// of each bin are taken in order into runs of at most 64 sites (a cell of
+// more than 64 sites gets ceil(count / 64) runs of its own).  The table's
+// length goes to *d_n_runs; it holds at most site_runs_bound(...).
+// d_bin_sites / d_bin_runs: n_bins ints of scratch each.
+long long site_runs_bound(int n_bins, long long n_sites);
+hipError_t launch_site_scan(const int* d_cell_counts, int n_bins, int* d_bin_sites,
+                            int* d_bin_runs, int* d_first, SiteRun* d_runs, int* d_n_runs,
+                            hipStream_t s);
+// Slots of the binned sites (offset from the bin centre, site id) and the
+// cell counts cleared for the next binning.
+hipError_t launch_site_scatter(long long n_sites, const double* centers, const double* axes,
+                               double o0, double o1, const Mat3& H, const Mat3& Hinv,
+                               const int m[3], const int* cell, const int* rank, const int* first,
+                               int* cell_counts, double* site_rel, int* site_id, hipStream_t s);
+// Per-site energies site_e [2 n] and/or gradients site_g [2 n][3] of the
+// binned sites (site_rel, site_id by slot); max_runs bounds the grid and
+// *d_n_runs (device) holds the run count.
+struct SiteArrays {
+  const double* site_rel;
+  const int* site_id;
+  double* site_e;          // [parts][part_stride]: part p of site k at p * part_stride + k
+  double* site_g;          // [parts][part_stride][3]
+  long long part_stride;   // >= 2 n
+  int parts;               // blocks per run (1..kMaxSiteParts), each with every parts-th batch
+};
+constexpr int kMaxSiteParts = 8;
+hipError_t launch_site(int precision, bool energy, bool grad, const Job* d_jobs,
+                       const SiteRun* d_runs, int max_runs, const int* d_n_runs,
+                       const long long* d_offsets, const float4* d_list32, const double* d_list64,
+                       const SiteArrays& a, hipStream_t s);
+// Pose sums (each site's parts added in part order): energy[i] = E_2i +
+// E_2i+1, grad_center = g (G_2i + G_2i+1), grad_axis = g (o0 G_2i + o1 G_2i+1),
+// g = grad_out[i] or 1 (outputs may be null); nonfinite results set flag 1.
+hipError_t launch_site_combine(int n_poses, const SiteArrays& a, double o0, double o1,
+                               const double* grad_out, double* energy, double* grad_center,
+                               double* grad_axis, int* bad, hipStream_t s);
+hipError_t launch_site_census(const Job* d_jobs, const SiteRun* d_runs, int max_runs,
+                              const int* d_n_runs, const long long* d_offsets,
+                              const double* d_list64, const double* site_rel,
+                              unsigned long long* pairs, hipStream_t s);
 hipError_t launch_legacy(int n_centers, int n_axes, double cutoff2, const double* centers,
                          const double* axes, const double* images, const double* offsets,
                          const double* coeff12, const double* coeff6, const double* shift,

@@ -35,50 +35,11 @@
 // Matrix cores are not used: the distances are a 16x16x4 FP32 matrix product
 // per 16 atoms x 16 orientations, but FP32 MFMA does not co-execute with the
 // VALU on gfx950 and such a kernel measured slower than this one.
-#include "sv_internal.h"
+#include "sv_device.h"
 #include <climits>
 
 namespace sv {
 namespace {
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Number of set bits of m in lanes below the calling lane.
-__device__ __forceinline__ int lanes_below(unsigned long long m) {
-  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-// Wave-uniform values loaded through pointers (vector loads) or computed by
-// VALU land in vector registers; readfirstlane moves them to scalar registers.
-__device__ __forceinline__ float uniform_f(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-__device__ __forceinline__ double uniform_d(double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8).
-// Map hardware block b to a logical block so that runs of kXcdRun consecutive
-// logical blocks (same tile, same candidate list) execute on one XCD and share
-// its L2, while the runs rotate over the XCDs: every XCD gets an even share of
-// each part of the work range, so batches sorted by cost stay balanced.  The
-// last incomplete round maps to itself.
-constexpr long long kXcdRun = 64;
-__device__ __forceinline__ long long xcd_block(long long b, long long nb) {
-  const long long round = 8 * kXcdRun;
-  if (b >= nb / round * round) return b;
-  const long long x = b % 8, i = b / 8;
-  return ((i / kXcdRun) * 8 + x) * kXcdRun + i % kXcdRun;
-}
 
 __device__ __forceinline__ int find_job(const long long* __restrict__ job_work, int n_jobs,
                                         long long w) {
@@ -133,17 +94,19 @@ __device__ __forceinline__ ShiftBox shift_box(const double d[3], const double hn
   return b;
 }
 
-// Cartesian offset of lattice image (d + n) and whether it lies within r2.
+// Cartesian offset of lattice image (d + n) and the class it falls in: 0 or 1
+// within r2 (class 0 when its squared distance is below split2), -1 beyond.
 // Written with explicit fma so the count and fill kernels (and both passes of
 // the fill kernel) round identically and agree on every borderline image.
-__device__ __forceinline__ bool image_inside(const double d[3], int na, int nb, int nc,
-                                             const double* __restrict__ H, double r2, double* x,
-                                             double* y, double* z) {
+__device__ __forceinline__ int image_class(const double d[3], int na, int nb, int nc,
+                                           const double* __restrict__ H, double r2,
+                                           double split2, double* x, double* y, double* z) {
   const double ua = d[0] + na, ub = d[1] + nb, uc = d[2] + nc;
   *x = fma(ua, H[0], fma(ub, H[3], uc * H[6]));
   *y = fma(ua, H[1], fma(ub, H[4], uc * H[7]));
   *z = fma(ua, H[2], fma(ub, H[5], uc * H[8]));
-  return fma(*x, *x, fma(*y, *y, *z * *z)) <= r2;
+  const double q = fma(*x, *x, fma(*y, *y, *z * *z));
+  return q <= r2 ? (q < split2 ? 0 : 1) : -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -173,6 +136,12 @@ __device__ __forceinline__ int list_block_scan(int v, int* total) {
   return before + x;
 }
 
+// Segment of (tile g, type t, class c): seg_base[g] + t * n_cls + c.  Jobs with
+// one class put every image of a type in class 0 (split2 = +inf).
+__device__ __forceinline__ double class_split2(const Job& J, int t) {
+  return J.n_cls > 1 ? J.split2[t] : INFINITY;
+}
+
 __global__ __launch_bounds__(kListThreads) void tile_count_kernel(const Job* __restrict__ jobs,
                                                                   const Tile* __restrict__ tiles,
                                                                   const long long* __restrict__ seg_base,
@@ -182,7 +151,8 @@ __global__ __launch_bounds__(kListThreads) void tile_count_kernel(const Job* __r
   const Job& J = jobs[T.job];
   const double r2 = T.list_r * T.list_r;
   for (int t = 0; t < J.n_types; ++t) {
-    int count = 0;
+    const double split2 = class_split2(J, t);
+    int count[2] = {0, 0};
     for (int a = J.type_start[t] + threadIdx.x; a < J.type_start[t + 1]; a += kListThreads) {
       double d[3];
 #pragma unroll
@@ -192,12 +162,16 @@ __global__ __launch_bounds__(kListThreads) void tile_count_kernel(const Job* __r
         for (int nb = b.lo[1]; nb <= b.hi[1]; ++nb)
           for (int nc = b.lo[2]; nc <= b.hi[2]; ++nc) {
             double x, y, z;
-            count += image_inside(d, na, nb, nc, J.H, r2, &x, &y, &z);
+            const int c = image_class(d, na, nb, nc, J.H, r2, split2, &x, &y, &z);
+            count[0] += c == 0;
+            count[1] += c == 1;
           }
     }
-    int total;
-    (void)list_block_scan(count, &total);
-    if (threadIdx.x == 0) counts[seg_base[g] + t] = total;
+    for (int c = 0; c < J.n_cls; ++c) {
+      int total;
+      (void)list_block_scan(count[c], &total);
+      if (threadIdx.x == 0) counts[seg_base[g] + (long long)t * J.n_cls + c] = total;
+    }
   }
 }
 
@@ -301,10 +275,12 @@ __global__ __launch_bounds__(kListThreads) void tile_fill_kernel(const Job* __re
   const int g = blockIdx.x;
   const Tile& T = tiles[g];
   const Job& J = jobs[T.job];
-  const double r2 = T.list_r * T.list_r;
+  const double r2 = T.list_r * T.list_r, scale = J.list_scale;
   for (int t = 0; t < J.n_types; ++t) {
-    long long base = offsets[seg_base[g] + t];
-    const long long seg_end = offsets[seg_base[g] + t + 1];
+    const double split2 = class_split2(J, t);
+    const long long seg = seg_base[g] + (long long)t * J.n_cls;
+    long long base[2] = {offsets[seg], J.n_cls > 1 ? offsets[seg + 1] : 0};
+    const long long seg_end[2] = {offsets[seg + 1], J.n_cls > 1 ? offsets[seg + 2] : 0};
     const int a_end = J.type_start[t + 1];
     for (int a0 = J.type_start[t]; a0 < a_end; a0 += kListThreads) {
       const int a = a0 + (int)threadIdx.x;
@@ -316,31 +292,35 @@ __global__ __launch_bounds__(kListThreads) void tile_fill_kernel(const Job* __re
         for (int k = 0; k < 3; ++k) d[k] = J.frac[3 * a + k] - T.frac[k];
         b = shift_box(d, J.hn, T.list_r);
       }
-      // Pass 1: how many images this thread emits.
-      int mine = 0;
+      // Pass 1: how many images of each class this thread emits.
+      int mine[2] = {0, 0};
       for (int na = b.lo[0]; na <= b.hi[0]; ++na)
         for (int nb = b.lo[1]; nb <= b.hi[1]; ++nb)
           for (int nc = b.lo[2]; nc <= b.hi[2]; ++nc) {
             double x, y, z;
-            mine += image_inside(d, na, nb, nc, J.H, r2, &x, &y, &z);
+            const int c = image_class(d, na, nb, nc, J.H, r2, split2, &x, &y, &z);
+            mine[0] += c == 0;
+            mine[1] += c == 1;
           }
-      int total;
-      long long slot = base + list_block_scan(mine, &total);
+      long long slot[2];
+      int total[2] = {0, 0};
+      for (int c = 0; c < J.n_cls; ++c) slot[c] = base[c] + list_block_scan(mine[c], &total[c]);
       // Pass 2: write them in atom order (deterministic list order), never
-      // past the end of this (tile, type) segment.
+      // past the end of their (tile, type, class) segment.
       for (int na = b.lo[0]; na <= b.hi[0]; ++na)
         for (int nb = b.lo[1]; nb <= b.hi[1]; ++nb)
           for (int nc = b.lo[2]; nc <= b.hi[2]; ++nc) {
             double x, y, z;
-            if (image_inside(d, na, nb, nc, J.H, r2, &x, &y, &z) && slot < seg_end) {
-              list32[slot] = make_float4((float)x, (float)y, (float)z, __int_as_float(a));
-              list64[3 * slot + 0] = x;
-              list64[3 * slot + 1] = y;
-              list64[3 * slot + 2] = z;
-              ++slot;
-            }
+            const int c = image_class(d, na, nb, nc, J.H, r2, split2, &x, &y, &z);
+            if (c < 0 || c >= J.n_cls || slot[c] >= seg_end[c]) continue;
+            const long long q = slot[c]++;
+            list32[q] = make_float4((float)(x * scale), (float)(y * scale), (float)(z * scale),
+                                    __int_as_float(a));
+            list64[3 * q + 0] = x;
+            list64[3 * q + 1] = y;
+            list64[3 * q + 2] = z;
           }
-      base += total;
+      for (int c = 0; c < J.n_cls; ++c) base[c] += total[c];
     }
   }
 }
@@ -897,351 +877,6 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
   write_pose(c, E, do_site, S);
 }
 
-// ---------------------------------------------------------------------------
-// Independent poses (center_i, director_i): energy and, with GRAD, the
-// analytic gradient.  Lanes are up to 64 poses of one tile; the block's
-// kPoseWaves waves split the tile's candidate list (64-entry pieces dealt
-// round-robin, read by broadcast from LDS) and their per-lane sums are added
-// in wave order at the end, so one chunk of poses fills several SIMDs.  With
-// r = image - center, t = r.u and g_s = dphi/d(d^2) of site s,
-//   dE/dc = 2 [u sum_s o_s G_s - sum_j r_j (g_0j + g_1j)]
-//   dE/du = 2 [u sum_s o_s^2 G_s - sum_j r_j (o_0 g_0j + o_1 g_1j)]
-// where G_s = sum_j g_sj; g = 0 beyond the cutoff and inside the contact floor
-// (the floored term is constant there), as in the FP64 reference formula.
-// ---------------------------------------------------------------------------
-struct PoseAcc {
-  double E, G0, G1, Rc[3], Ru[3], Gc[3], Gu[3];
-};
-constexpr int kPoseWaves = 4;
-constexpr int kPoseAccN = 15;   // doubles in a PoseAcc (E, G0, G1, Rc, Ru, Gc, Gu)
-
-// FP64 pair energy and gradient contribution with explicit d = o u - r.
-// FAST takes 1/d^2 from v_rcp_f64 and one Newton step instead of the IEEE
-// division of the reference form; the pose kernels use it in both forms (no
-// reference counterpart asks for IEEE rounding there).
-template <bool FAST = false>
-__device__ __forceinline__ void pose_pair64(double o, double ux, double uy, double uz, double rx,
-                                            double ry, double rz, double rc2, double c12,
-                                            double c6, double sh, double fl2, bool grad,
-                                            PoseAcc& a) {
-  const double dx = o * ux - rx, dy = o * uy - ry, dz = o * uz - rz;
-  const double d2 = dx * dx + dy * dy + dz * dz;
-  if (d2 < rc2) {
-    const bool floored = d2 < fl2;
-    const double safe = fmax(d2, fl2);
-    const double inv = FAST ? inv_fast(safe) : 1.0 / safe;
-    const double i6 = inv * inv * inv;
-    a.E += c12 * i6 * i6 - c6 * i6 + sh;
-    if (grad && !floored) {
-      const double g = (-6.0 * c12 * i6 * i6 + 3.0 * c6 * i6) * inv;
-      a.Gc[0] += g * dx; a.Gc[1] += g * dy; a.Gc[2] += g * dz;
-      a.Gu[0] += g * o * dx; a.Gu[1] += g * o * dy; a.Gu[2] += g * o * dz;
-    }
-  }
-}
-
-template <bool MIXED, bool GRAD>
-__global__ __launch_bounds__(64 * kPoseWaves) void pose_kernel(const Job* __restrict__ jobs,
-                                                               const SiteChunk* __restrict__ chunks,
-                                                               const int* __restrict__ n_chunks,
-                                                               const long long* __restrict__ offsets,
-                                                               const float4* __restrict__ list32,
-                                                               const double* __restrict__ list64) {
-  __shared__ float4 s_e_all[kPoseWaves][64];
-  __shared__ double s_d_all[kPoseWaves][3][64];
-  __shared__ double s_acc[kPoseWaves - 1][GRAD ? kPoseAccN : 1][64];
-  // A chunk table built on the device is launched with a grid that bounds its
-  // length; blocks past the actual count have nothing to do.
-  if (n_chunks && (int)blockIdx.x >= *n_chunks) return;
-  const SiteChunk ch = chunks[blockIdx.x];
-  const Job& J = jobs[ch.job];
-  const int lane = threadIdx.x & 63;
-  const int wv = uniform(threadIdx.x >> 6);
-  float4* s_e = s_e_all[wv];
-  double (*s_d)[64] = s_d_all[wv];
-  const bool ok = lane < ch.count;
-  const int slot = ch.first + (ok ? lane : 0);
-  const double cx = J.center_rel[3 * slot], cy = J.center_rel[3 * slot + 1],
-               cz = J.center_rel[3 * slot + 2];
-  const double ux = J.pose_axes64[3 * slot], uy = J.pose_axes64[3 * slot + 1],
-               uz = J.pose_axes64[3 * slot + 2];
-  float cxf = (float)cx, cyf = (float)cy, czf = (float)cz;
-  float uxf = (float)ux, uyf = (float)uy, uzf = (float)uz;
-  // keep the FP32 copies in registers: left to itself the compiler converts
-  // them again on every candidate
-  asm volatile("" : "+v"(cxf), "+v"(cyf), "+v"(czf), "+v"(uxf), "+v"(uyf), "+v"(uzf));
-  const double o0 = J.o0, o1 = J.o1, rc2 = J.rc2;
-  // FP32 pairs work in units of 1/s (s = FP32 1/rc^2): x = s d^2 is clamped to
-  // [0, 1] by the clamp bit of the fma that forms it, so a pair beyond the
-  // cutoff contributes the clamped term, which the +shift added for every
-  // FP32 candidate cancels (up to ~3e-8 K per pair from the rounding of s).
-  const float sc = J.inv_rc2f, m2o0 = J.m2o0xf, m2o1 = J.m2o1xf;
-  float o0s = J.o0sxf, o1s = J.o1sxf;
-  // held in VGPRs: an fma reads at most one scalar register
-  asm volatile("" : "+v"(o0s), "+v"(o1s));
-  const float o0f = (float)o0, o1f = (float)o1;
-  const long long seg0 = J.seg_base + (long long)ch.tile * J.n_types;
-  PoseAcc a = {};
-  for (int t = 0; t < J.n_types; ++t) {
-    const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
-    const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
-    // c6/d^6 (sigma^6/d^6 - 1) = c6 s^3 i^3 (sigma^6 s^3 i^3 - 1), i = 1/x;
-    // dphi/d(d^2) = 3 c6 s^4 i^4 (1 - 2 sigma^6 s^3 i^3)
-    const double c6x = J.c6x[t], k3 = 3.0 * c6x * (double)sc;
-    const float s6x = J.s6xf[t], m2s6x = -2.0f * J.s6xf[t];
-    // near-contact candidates, and all within reach of the contact floor, are FP64
-    const float near2 = fmaxf(J.near2f[t], J.lo2f[t]);
-    long long n_fp = 0;   // FP32 candidates of this type (each gets +shift per site)
-    for (long long base = beg + 64 * wv; base < end; base += 64 * kPoseWaves) {
-      const long long idx = base + lane;
-      if (idx < end) {
-        // the FP64 copy serves the FP64 form and the mixed form's near terms
-        if (MIXED) s_e[lane] = list32[idx];
-        s_d[0][lane] = list64[3 * idx];
-        s_d[1][lane] = list64[3 * idx + 1];
-        s_d[2][lane] = list64[3 * idx + 2];
-      }
-      wave_sync();
-      const int n = (int)(end - base < 64 ? end - base : 64);
-      if (MIXED) {
-        // FP32 partials over at most 16 atoms, then FP64.
-        n_fp += n;
-        for (int q0 = 0; q0 < n; q0 += 16) {
-          const int q1 = q0 + 16 < n ? q0 + 16 : n;
-          float aE = 0.f, aG0 = 0.f, aG1 = 0.f, aRc0 = 0.f, aRc1 = 0.f, aRc2 = 0.f, aRu0 = 0.f,
-                aRu1 = 0.f, aRu2 = 0.f;
-          for (int q = q0; q < q1; ++q) {
-            const float4 e = s_e[q];
-            const float rx = e.x - cxf, ry = e.y - cyf, rz = e.z - czf;
-            const float rr0 = rx * rx + ry * ry + rz * rz;
-            const float tt = rx * uxf + ry * uyf + rz * uzf;
-            const bool near = rr0 < near2;
-            // a near candidate's FP32 terms clamp like pairs beyond the cutoff
-            // (and are cancelled the same way); its FP64 terms follow below
-            const float rr = near ? 3.0e38f : rr0;
-            const float x0 = __builtin_amdgcn_fmed3f(fmaf(tt, m2o0, fmaf(rr, sc, o0s)), 0.0f, 1.0f);
-            const float x1 = __builtin_amdgcn_fmed3f(fmaf(tt, m2o1, fmaf(rr, sc, o1s)), 0.0f, 1.0f);
-            const float i0 = __builtin_amdgcn_rcpf(x0), i1 = __builtin_amdgcn_rcpf(x1);
-            const float i03 = i0 * i0 * i0, i13 = i1 * i1 * i1;
-            aE = fmaf(i03, fmaf(i03, s6x, -1.0f), aE);
-            aE = fmaf(i13, fmaf(i13, s6x, -1.0f), aE);
-            if (GRAD) {
-              // no gradient beyond the cutoff (x clamped to 1)
-              const float g0 = x0 < 1.0f ? i03 * i0 * fmaf(m2s6x, i03, 1.0f) : 0.0f;
-              const float g1 = x1 < 1.0f ? i13 * i1 * fmaf(m2s6x, i13, 1.0f) : 0.0f;
-              aG0 += g0;
-              aG1 += g1;
-              const float w = g0 + g1, v = fmaf(o0f, g0, o1f * g1);
-              aRc0 = fmaf(w, rx, aRc0); aRc1 = fmaf(w, ry, aRc1); aRc2 = fmaf(w, rz, aRc2);
-              aRu0 = fmaf(v, rx, aRu0); aRu1 = fmaf(v, ry, aRu1); aRu2 = fmaf(v, rz, aRu2);
-            }
-            if (near) {
-              const double qx = s_d[0][q] - cx, qy = s_d[1][q] - cy, qz = s_d[2][q] - cz;
-              pose_pair64<true>(o0, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
-              pose_pair64<true>(o1, ux, uy, uz, qx, qy, qz, rc2, c12, c6, sh, fl2, GRAD, a);
-            }
-          }
-          a.E = fma(c6x, (double)aE, a.E);
-          if (GRAD) {
-            const double k = k3;
-            a.G0 = fma(k, (double)aG0, a.G0);
-            a.G1 = fma(k, (double)aG1, a.G1);
-            a.Rc[0] = fma(k, (double)aRc0, a.Rc[0]);
-            a.Rc[1] = fma(k, (double)aRc1, a.Rc[1]);
-            a.Rc[2] = fma(k, (double)aRc2, a.Rc[2]);
-            a.Ru[0] = fma(k, (double)aRu0, a.Ru[0]);
-            a.Ru[1] = fma(k, (double)aRu1, a.Ru[1]);
-            a.Ru[2] = fma(k, (double)aRu2, a.Ru[2]);
-          }
-        }
-      } else {
-        for (int q = 0; q < n; ++q) {
-          const double rx = s_d[0][q] - cx, ry = s_d[1][q] - cy, rz = s_d[2][q] - cz;
-          pose_pair64<true>(o0, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2, GRAD, a);
-          pose_pair64<true>(o1, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2, GRAD, a);
-        }
-      }
-      wave_sync();
-    }
-    if (MIXED) a.E = fma((double)(2 * n_fp), sh, a.E);
-  }
-  // Waves 1.. hand their sums to wave 0, which adds them in wave order.
-  if (wv > 0) {
-    double(*o)[64] = s_acc[wv - 1];
-    o[0][lane] = a.E;
-    if (GRAD) {
-      o[1][lane] = a.G0;
-      o[2][lane] = a.G1;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        o[3 + k][lane] = a.Rc[k];
-        o[6 + k][lane] = a.Ru[k];
-        o[9 + k][lane] = a.Gc[k];
-        o[12 + k][lane] = a.Gu[k];
-      }
-    }
-  }
-  __syncthreads();
-  if (wv > 0 || !ok) return;
-#pragma unroll
-  for (int w = 0; w < kPoseWaves - 1; ++w) {
-    const double(*o)[64] = s_acc[w];
-    a.E += o[0][lane];
-    if (GRAD) {
-      a.G0 += o[1][lane];
-      a.G1 += o[2][lane];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        a.Rc[k] += o[3 + k][lane];
-        a.Ru[k] += o[6 + k][lane];
-        a.Gc[k] += o[9 + k][lane];
-        a.Gu[k] += o[12 + k][lane];
-      }
-    }
-  }
-  const int cid = J.center_id[slot];
-  // A pose with a nonfinite center or a nonfinite / zero director (only device
-  // binned poses can carry one; it is flagged there) gets NaN results rather
-  // than the sum over the no pairs its NaN distances select.
-  const double unn = ux * ux + uy * uy + uz * uz;
-  const bool valid = isfinite(cx) && isfinite(cy) && isfinite(cz) && isfinite(unn) && unn > 0.0;
-  if (!valid) {
-    a.E = __builtin_nan("");
-    a.G0 = a.G1 = a.E;
-  }
-  if (!isfinite(a.E)) atomicOr(J.bad, 1);
-  if (J.pose_e) J.pose_e[cid] = a.E;
-  if (GRAD) {
-    const double go = J.grad_out ? J.grad_out[cid] : 1.0;
-    const double sc = o0 * a.G0 + o1 * a.G1, su = o0 * o0 * a.G0 + o1 * o1 * a.G1;
-    const double u[3] = {ux, uy, uz};
-    for (int k = 0; k < 3; ++k) {
-      const double gc = 2.0 * (u[k] * sc - a.Rc[k] + a.Gc[k]);
-      const double gu = 2.0 * (u[k] * su - a.Ru[k] + a.Gu[k]);
-      if (J.grad_c) J.grad_c[3 * (long long)cid + k] = go * gc;
-      if (J.grad_u) J.grad_u[3 * (long long)cid + k] = go * gu;
-      if (!isfinite(gc) || !isfinite(gu)) atomicOr(J.bad, 1);
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Device binning of independent poses into a fixed grid of m0 x m1 x m2 tiles
-// (bins of the fractional cell).  Poses stay in device memory: bin + rank by
-// one atomic per pose, an exclusive scan over the bins that also cuts every
-// bin into <= 64-pose chunks, and a scatter into tile-sorted slots.  The
-// order of poses inside a bin depends on atomic timing, which changes no
-// result: each pose is evaluated by its own lane over its bin's list.
-// ---------------------------------------------------------------------------
-// x - floor(x): the same value as fmod(x, 1) (+1 when negative) -- both
-// subtractions are exact or round alike -- at one v_floor_f64.
-__device__ __forceinline__ double wrap01_dev(double x) { return x - floor(x); }
-
-__global__ __launch_bounds__(256) void pose_bin_kernel(int n, const double* __restrict__ centers,
-                                                       const double* __restrict__ axes, Mat3 Hinv,
-                                                       int m0, int m1, int m2,
-                                                       int* __restrict__ bin,
-                                                       int* __restrict__ rank,
-                                                       int* __restrict__ counts,
-                                                       int* __restrict__ bad) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double c[3] = {centers[3 * i], centers[3 * i + 1], centers[3 * i + 2]};
-  const double u0 = axes[3 * i], u1 = axes[3 * i + 1], u2 = axes[3 * i + 2];
-  const double nn = u0 * u0 + u1 * u1 + u2 * u2;
-  bool ok = isfinite(c[0]) && isfinite(c[1]) && isfinite(c[2]) && isfinite(nn) && nn > 0.0;
-  const int m[3] = {m0, m1, m2};
-  double f[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    f[k] = wrap01_dev(c[0] * Hinv.m[k] + c[1] * Hinv.m[3 + k] + c[2] * Hinv.m[6 + k]);
-    ok = ok && f[k] >= 0.0 && f[k] <= 1.0;   // false for an overflowed (nonfinite) product
-  }
-  int idx[3] = {0, 0, 0};
-  if (ok) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int q = (int)(f[k] * m[k]);
-      idx[k] = q < m[k] - 1 ? q : m[k] - 1;
-    }
-  } else {
-    atomicOr(bad, 2);
-  }
-  const int b = (idx[0] * m1 + idx[1]) * m2 + idx[2];
-  bin[i] = b;
-  rank[i] = atomicAdd(&counts[b], 1);
-}
-
-// One block: exclusive scan of the bin counts into first[] and the chunk
-// table (job, bin, first slot, <= 64 poses), whose length goes to *n_chunks.
-__global__ __launch_bounds__(1024) void pose_tile_scan_kernel(const int* __restrict__ counts,
-                                                              int n_bins, int job,
-                                                              int* __restrict__ first,
-                                                              SiteChunk* __restrict__ chunks,
-                                                              int* __restrict__ n_chunks) {
-  __shared__ int sa[1024], sb[1024];
-  const int t = threadIdx.x;
-  int carry_a = 0, carry_b = 0;
-  for (int base = 0; base < n_bins; base += 1024) {
-    const int i = base + t;
-    const int c = i < n_bins ? counts[i] : 0;
-    const int k = (c + 63) >> 6;
-    sa[t] = c;
-    sb[t] = k;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int va = t >= off ? sa[t - off] : 0, vb = t >= off ? sb[t - off] : 0;
-      __syncthreads();
-      sa[t] += va;
-      sb[t] += vb;
-      __syncthreads();
-    }
-    const int ea = carry_a + sa[t] - c, eb = carry_b + sb[t] - k;
-    if (i < n_bins) {
-      first[i] = ea;
-      for (int q = 0; q < k; ++q) {
-        const int left = c - 64 * q;
-        chunks[eb + q] = SiteChunk{job, i, ea + 64 * q, left < 64 ? left : 64};
-      }
-    }
-    carry_a += sa[1023];
-    carry_b += sb[1023];
-    __syncthreads();
-  }
-  if (t == 0) *n_chunks = carry_b;
-}
-
-// Tile-sorted slots: Cartesian offset from the bin centre (from fractional
-// differences, as for host-built pose sets), caller index, bin, director.
-__global__ __launch_bounds__(256) void pose_scatter_kernel(
-    int n, const double* __restrict__ centers, const double* __restrict__ axes, Mat3 H, Mat3 Hinv,
-    int m0, int m1, int m2, const int* __restrict__ bin, const int* __restrict__ rank,
-    const int* __restrict__ first, double* __restrict__ center_rel, int* __restrict__ center_id,
-    int* __restrict__ tile_of, double* __restrict__ axes64, float* __restrict__ axes32) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int b = bin[i];
-  const int slot = first[b] + rank[i];
-  const int idx[3] = {b / (m1 * m2), (b / m2) % m1, b % m2};
-  const int m[3] = {m0, m1, m2};
-  const double c[3] = {centers[3 * i], centers[3 * i + 1], centers[3 * i + 2]};
-  double d[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const double f = wrap01_dev(c[0] * Hinv.m[k] + c[1] * Hinv.m[3 + k] + c[2] * Hinv.m[6 + k]);
-    d[k] = f - (idx[k] + 0.5) / m[k];
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    center_rel[3 * (long long)slot + k] = d[0] * H.m[k] + d[1] * H.m[3 + k] + d[2] * H.m[6 + k];
-    const double u = axes[3 * i + k];
-    axes64[3 * (long long)slot + k] = u;
-    axes32[3 * (long long)slot + k] = (float)u;
-  }
-  center_id[slot] = i;
-  tile_of[slot] = b;
-}
-
 // Exact count of in-cutoff (site, image) pairs: the pair-eval census that the
 // throughput metric is quoted in.  Same traversal as field_fp64_kernel.
 __global__ __launch_bounds__(256) void census_kernel(const Job* __restrict__ jobs, int n_jobs,
@@ -1374,29 +1009,6 @@ __global__ __launch_bounds__(256) void legacy_kernel(int n_centers, int n_axes, 
   }
 }
 
-// Points the pose job at this call's arrays and clears `zero[0..n_zero)`
-// (the bin counts of a binning pass, or nothing).
-__global__ __launch_bounds__(256) void job_patch_kernel(Job* __restrict__ J, const double* center_rel,
-                                 const int* center_id, const int* tile_of, const double* axes64,
-                                 const float* axes32, int n, const double* grad_out,
-                                 double* pose_e, double* grad_c, double* grad_u,
-                                 int* __restrict__ zero, int n_zero) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n_zero) zero[i] = 0;
-  if (i == 0) {
-    J->center_rel = center_rel;
-    J->center_id = center_id;
-    J->tile_of = tile_of;
-    J->pose_axes64 = axes64;
-    J->pose_axes32 = axes32;
-    J->n_centers = n;
-    J->grad_out = grad_out;
-    J->pose_e = pose_e;
-    J->grad_c = grad_c;
-    J->grad_u = grad_u;
-  }
-}
-
 __global__ void zero_kernel(int* __restrict__ p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -1417,16 +1029,6 @@ __global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int n) {
 hipError_t launch_zero(int* p, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_job_patch(Job* d_job, const double* center_rel, const int* center_id,
-                            const int* tile_of, const double* axes64, const float* axes32, int n,
-                            const double* grad_out, double* pose_e, double* grad_c,
-                            double* grad_u, int* zero, int n_zero, hipStream_t s) {
-  const int blocks = n_zero > 256 ? (n_zero + 255) / 256 : 1;
-  hipLaunchKernelGGL(job_patch_kernel, dim3(blocks), dim3(256), 0, s, d_job, center_rel, center_id,
-                     tile_of, axes64, axes32, n, grad_out, pose_e, grad_c, grad_u, zero, n_zero);
   return hipGetLastError();
 }
 
@@ -1522,57 +1124,6 @@ hipError_t launch_marginal(const Job* d_jobs, int n_jobs, const long long* d_job
   if (n_total <= 0) return hipSuccess;
   hipLaunchKernelGGL(marginal_kernel, dim3((unsigned)n_total), dim3(64), 0, s, d_jobs, n_jobs,
                      d_job_first, n_total);
-  return hipGetLastError();
-}
-
-hipError_t launch_pose(int precision, bool grad, const Job* d_jobs, const SiteChunk* d_chunks,
-                       int n_chunks, const int* d_n_chunks, const long long* d_offsets,
-                       const float4* d_list32, const double* d_list64, hipStream_t s) {
-  if (n_chunks <= 0) return hipSuccess;
-  const dim3 g(n_chunks), b(64 * kPoseWaves);
-  if (precision == PREC_MIXED) {
-    if (grad)
-      hipLaunchKernelGGL((pose_kernel<true, true>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,
-                         d_offsets, d_list32, d_list64);
-    else
-      hipLaunchKernelGGL((pose_kernel<true, false>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,
-                         d_offsets, d_list32, d_list64);
-  } else {
-    if (grad)
-      hipLaunchKernelGGL((pose_kernel<false, true>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,
-                         d_offsets, d_list32, d_list64);
-    else
-      hipLaunchKernelGGL((pose_kernel<false, false>), g, b, 0, s, d_jobs, d_chunks, d_n_chunks,
-                         d_offsets, d_list32, d_list64);
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_pose_bin(int n, const double* d_centers, const double* d_axes, const Mat3& Hinv,
-                           const int m[3], int* d_bin, int* d_rank, int* d_counts, int* d_bad,
-                           hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(pose_bin_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                     d_centers, d_axes, Hinv, m[0], m[1], m[2], d_bin, d_rank, d_counts, d_bad);
-  return hipGetLastError();
-}
-
-hipError_t launch_pose_tile_scan(const int* d_counts, int n_bins, int job, int* d_first,
-                                 SiteChunk* d_chunks, int* d_n_chunks, hipStream_t s) {
-  hipLaunchKernelGGL(pose_tile_scan_kernel, dim3(1), dim3(1024), 0, s, d_counts, n_bins, job,
-                     d_first, d_chunks, d_n_chunks);
-  return hipGetLastError();
-}
-
-hipError_t launch_pose_scatter(int n, const double* d_centers, const double* d_axes, const Mat3& H,
-                               const Mat3& Hinv, const int m[3], const int* d_bin,
-                               const int* d_rank, const int* d_first, double* d_center_rel,
-                               int* d_center_id, int* d_tile_of, double* d_axes64, float* d_axes32,
-                               hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(pose_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                     d_centers, d_axes, H, Hinv, m[0], m[1], m[2], d_bin, d_rank, d_first,
-                     d_center_rel, d_center_id, d_tile_of, d_axes64, d_axes32);
   return hipGetLastError();
 }
 

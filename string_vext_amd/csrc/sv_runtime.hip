@@ -44,6 +44,19 @@ double wrap01(double x) {
   return r;
 }
 
+// u / |u| with |u| = sqrt((x^2 + y^2) + z^2), each operation rounded on its
+// own: the normalization numpy applies to the directors of a pose set
+// (axes / np.linalg.norm(axes, axis=1, keepdims=True), reference
+// native.py:34).  False for a nonfinite or zero director.
+bool unit_director(const double* u, double* out) {
+#pragma clang fp contract(off)
+  const double nn = (u[0] * u[0] + u[1] * u[1]) + u[2] * u[2];
+  if (!std::isfinite(nn) || nn <= 0.0) return false;
+  const double norm = std::sqrt(nn);
+  for (int k = 0; k < 3; ++k) out[k] = u[k] / norm;
+  return true;
+}
+
 bool invert3(const double* H, double* inv, double* det_out) {
   const double a = H[0], b = H[1], c = H[2], d = H[3], e = H[4], f = H[5], g = H[6], h = H[7],
                i = H[8];
@@ -107,6 +120,9 @@ void plan_fill_job(const Plan& P, Job& J) {
   J.inv_rc2f = (float)(1.0 / J.rc2);
   J.m2o0sf = (float)(-2.0 * P.o0 / J.rc2);
   J.m2o1sf = (float)(-2.0 * P.o1 / J.rc2);
+  J.nsite2f = P.d_nsite2f.p;
+  J.n_cls = 1;
+  J.list_scale = 1.0;
 }
 
 namespace {
@@ -136,6 +152,64 @@ void choose_tiles(const Plan& P, long long n_centers, int grid, int tile_centers
   }
 }
 
+// Bins of a site grid for about n_sites uniformly spread sites: the counts per
+// lattice direction that minimize the expected candidate work per site,
+//   sum_n Poisson(n; lambda) sum_{chunks j} (s_k r_b + rc)^3 / lambda,
+// with lambda = n_sites / bins, k = ceil(n / 64) chunks per bin, r_b the bin's
+// circumradius and s_k the mean bounding radius of a chunk of k in units of
+// r_b (0.82, 0.72, 0.72, 0.62 for k = 1, 2, 3, >= 4: sampled for the octant
+// order and even split, over rhombic and skew cells).  Searched about the
+// counts that give bins as thick as they are wide.
+void choose_site_bins(const Plan& P, long long n_sites, int m_out[3]) {
+  const double rho = (double)std::max(1LL, n_sites) / P.volume;
+  const double rc = P.cutoff;
+  int guess[3];
+  const double edge = std::cbrt(96.0 / rho);
+  for (int k = 0; k < 3; ++k) guess[k] = std::max(1, (int)std::lround(1.0 / P.hn[k] / edge));
+  double best = INFINITY;
+  for (int k = 0; k < 3; ++k) m_out[k] = guess[k];
+  auto range = [](int g, int* lo, int* hi) {
+    *lo = std::max(1, (int)std::floor(g * 0.55));
+    *hi = std::min(1024, std::max(*lo, (int)std::ceil(g * 1.6)));
+  };
+  int lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) range(guess[k], &lo[k], &hi[k]);
+  const double shrink[5] = {1.0, 0.82, 0.72, 0.72, 0.62};
+  for (int a = lo[0]; a <= hi[0]; ++a)
+    for (int b = lo[1]; b <= hi[1]; ++b)
+      for (int c = lo[2]; c <= hi[2]; ++c) {
+        const double bins = (double)a * b * c;
+        if (bins > (1 << 18)) continue;
+        const double lambda = (double)n_sites / bins;
+        double rb = 0.0;
+        for (int sb = -1; sb <= 1; sb += 2)
+          for (int sc = -1; sc <= 1; sc += 2) {
+            double r2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+              const double x = P.H[k] / a + sb * P.H[3 + k] / b + sc * P.H[6 + k] / c;
+              r2 += x * x;
+            }
+            rb = std::max(rb, 0.5 * std::sqrt(r2));
+          }
+        // expected work over the Poisson count of one bin
+        const int n_max = (int)(lambda + 8.0 * std::sqrt(lambda) + 64.0);
+        double work = 0.0, logp = -lambda;   // log P(0)
+        for (int n = 0; n <= n_max; ++n) {
+          if (n > 0) logp += std::log(lambda) - std::log((double)n);
+          const int chunks = (n + 63) / 64;
+          if (chunks) {
+            const double r = shrink[std::min(chunks, 4)] * rb + rc;
+            work += std::exp(logp) * chunks * r * r * r;
+          }
+        }
+        const double cost = work / lambda;
+        if (cost < best) {
+          best = cost;
+          m_out[0] = a, m_out[1] = b, m_out[2] = c;
+        }
+      }
+}
+
 int build_poses(Plan* P, int n, const double* frac_in, int n_axes, const double* axes, int grid,
                 int tile_centers, Poses** out) {
   if (n < 0 || n_axes < 1) return fail(1, "n_centers >= 0 and n_axes >= 1 required");
@@ -150,11 +224,9 @@ int build_poses(Plan* P, int n, const double* frac_in, int n_axes, const double*
   S->n_centers = n;
   S->n_axes = n_axes;
   S->n_groups = (n_axes + 63) / 64;
-  S->axes.assign(axes, axes + 3 * (size_t)n_axes);
+  S->axes.resize(3 * (size_t)n_axes);
   for (int k = 0; k < n_axes; ++k) {
-    const double* u = axes + 3 * k;
-    const double nn = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    if (!std::isfinite(nn) || nn <= 0.0) {
+    if (!unit_director(axes + 3 * k, &S->axes[3 * k])) {
       delete S;
       return fail(1, "finite nonzero directors required");
     }
@@ -258,8 +330,9 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
     J.tile_of = S->d_tile_of.p;
     J.axes64 = S->d_axes64.p;
     J.axes32 = S->d_axes32.p;
-    J.pose_axes64 = S->d_pose_axes64.p;
-    J.pose_axes32 = S->d_pose_axes32.p;
+    J.n_cls = S->n_cls;
+    J.list_scale = S->list_scale;
+    J.split2 = S->split2;
     J.n_centers = S->n_centers;
     J.n_axes = S->n_axes;
     J.n_groups = S->n_groups;
@@ -294,12 +367,12 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
       Tile T = S->tiles[t];
       T.job = j;
       B->tiles.push_back(T);
-      B->seg_base.push_back(segs + (long long)t * P->n_types);
-      if (J.site32 || S->per_pose)
+      B->seg_base.push_back(segs + (long long)t * P->n_types * S->n_cls);
+      if (J.site32)
         for (int first = 0; first < T.n_members; first += 64)
           B->chunks.push_back({j, (int)t, T.first_member + first, std::min(64, T.n_members - first)});
     }
-    segs += (long long)S->tiles.size() * P->n_types;
+    segs += (long long)S->tiles.size() * P->n_types * S->n_cls;
     work += (long long)S->n_centers * S->n_groups;
     B->job_first[j] = centers;
     centers += S->n_centers;
@@ -381,14 +454,17 @@ int batch_lists(Batch* B, bool timed) {
 
 }  // namespace
 
-// A pose grid of m[0] x m[1] x m[2] bins over the cell.  Every bin is a tile
-// whose list radius covers the whole bin (the circumscribed sphere of the
-// sub-cell a/m0, b/m1, c/m2 about its centre); the lists are built here, once.
-int pose_grid_build(Plan* P, const int m_in[3], PoseGrid** out) {
+// A grid of m[0] x m[1] x m[2] bins over the cell.  Every bin is a tile whose
+// list radius covers the whole bin (the circumscribed sphere of the sub-cell
+// a/m0, b/m1, c/m2 about its centre).  Site lists add rc, are split per type
+// at rsite + bin radius (class 0: images a member site can bring within its
+// near-contact radius) and hold FP32 coordinates in units of rc; center lists
+// add rc + max|o|.  The lists are built here, once.
+int pose_grid_build(Plan* P, const int m_in[3], bool site_lists, PoseGrid** out) {
   for (int k = 0; k < 3; ++k)
     if (m_in[k] < 1 || m_in[k] > 1024) return fail(1, "tile counts must be in 1..1024");
   const long long nb = (long long)m_in[0] * m_in[1] * m_in[2];
-  if (nb > (1 << 22)) return fail(1, "pose grid would need more than 2^22 tiles");
+  if (nb > (1 << 18)) return fail(1, "pose grid would need more than 2^18 tiles");
   hipError_t e;
   if ((e = hipSetDevice(P->device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
   auto G = new PoseGrid();
@@ -409,12 +485,37 @@ int pose_grid_build(Plan* P, const int m_in[3], PoseGrid** out) {
     }
   radius *= 1.0 + 1e-12;
   G->radius = radius;
+  {
+    // sub-grid bits along the lattice axes by bin edge length, longest first
+    int ax[3] = {0, 1, 2};
+    double len[3];
+    for (int k = 0; k < 3; ++k)
+      len[k] = std::sqrt(P->H[3 * k] * P->H[3 * k] + P->H[3 * k + 1] * P->H[3 * k + 1] +
+                         P->H[3 * k + 2] * P->H[3 * k + 2]) / G->m[k];
+    std::stable_sort(ax, ax + 3, [&](int a, int b) { return len[a] > len[b]; });
+    G->order = ax[0] | ax[1] << 2 | ax[2] << 4;
+  }
+  std::vector<double> split2(P->n_types);
+  for (int t = 0; t < P->n_types; ++t) {
+    const double r = P->rsite[t] + radius + 1e-3;
+    split2[t] = r * r;
+  }
+  if ((e = G->split2.upload(split2.data(), split2.size(), P->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
+    delete G;
+    return hip_fail(e, "site grid tables");
+  }
   auto S = new Poses();
   G->S = S;
   S->plan = P;
   S->n_axes = 1;
   S->n_groups = 1;
   S->per_pose = true;
+  if (site_lists) {
+    S->n_cls = 2;
+    S->list_scale = 1.0 / P->cutoff;
+    S->split2 = G->split2.p;
+  }
   S->tiles.resize(G->n_bins);
   for (int b = 0; b < G->n_bins; ++b) {
     Tile& T = S->tiles[b];
@@ -422,16 +523,21 @@ int pose_grid_build(Plan* P, const int m_in[3], PoseGrid** out) {
     const int idx[3] = {b / (G->m[1] * G->m[2]), (b / G->m[2]) % G->m[1], b % G->m[2]};
     for (int k = 0; k < 3; ++k) T.frac[k] = (idx[k] + 0.5) / G->m[k];
     T.radius = radius;
-    T.list_r = radius + P->keep_r + 1e-3;
+    T.list_r = radius + (site_lists ? P->cutoff : P->keep_r) + 1e-3;
   }
   Plan* pl = P;
   Poses* po = S;
   int rc = batch_build(1, &pl, &po, PREC_FP64, nullptr, nullptr, nullptr, nullptr, nullptr,
                        nullptr, &G->B);
   if (!rc) rc = batch_lists(G->B, false);
-  if (!rc && ((e = G->counts.reserve(G->n_bins)) != hipSuccess ||
-              (e = G->first.reserve(G->n_bins)) != hipSuccess ||
-              (e = G->n_chunks.reserve(1)) != hipSuccess ||
+  if (!rc && ((e = G->counts.reserve(64 * (size_t)G->n_bins)) != hipSuccess ||
+              (e = G->first.reserve(64 * (size_t)G->n_bins)) != hipSuccess ||
+              (e = G->n_runs.reserve(1)) != hipSuccess ||
+              (e = G->bin_sites.reserve(G->n_bins)) != hipSuccess ||
+              (e = G->bin_runs.reserve(G->n_bins)) != hipSuccess ||
+              // the binning pass expects cleared cell counts and clears them after use
+              (e = hipMemsetAsync(G->counts.p, 0, 64 * sizeof(int) * (size_t)G->n_bins,
+                                  G->B->stream)) != hipSuccess ||
               (e = hipStreamSynchronize(G->B->stream)) != hipSuccess))
     rc = hip_fail(e, "pose grid buffers");
   if (rc) {
@@ -568,6 +674,8 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
     P->c6x.push_back(c6 * s * s * s);
     P->s6xf.push_back((float)(c12 / c6 * s * s * s));
     float near2 = 0.0f;
+    // the site form of independent poses takes the same radii about one site
+    double r_site = std::sqrt(fl) + 1e-3;
     if (near_energy > 0.0) {
       // Repulsive-branch distance where 4 eps (x^2 - x) = near_energy, x = (sigma/r)^6.
       const double x = 0.5 * (1.0 + std::sqrt(1.0 + near_energy / me));
@@ -584,8 +692,11 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
       }
       const double rn = std::max(r_energy, hi) + omax;
       near2 = (float)(rn * rn);
+      r_site = std::max(r_site, std::max(r_energy, hi));
     }
     P->near2f.push_back(near2);
+    P->rsite.push_back(r_site);
+    P->nsite2f.push_back((float)((r_site / cutoff) * (r_site / cutoff)));
     const double lo = std::sqrt(fl) + omax + 1e-3;
     P->lo2f.push_back((float)(lo * lo));
   }
@@ -614,6 +725,7 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
       (e = P->d_s6sf.upload(P->s6sf.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = P->d_c6x.upload(P->c6x.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = P->d_s6xf.upload(P->s6xf.data(), P->n_types, P->stream)) != hipSuccess ||
+      (e = P->d_nsite2f.upload(P->nsite2f.data(), P->n_types, P->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
     delete P;
     return hip_fail(e, "plan upload");
@@ -644,7 +756,7 @@ extern "C" {
 
 const char* sv_last_error(void) { return t_error.c_str(); }
 
-int sv_abi_version(void) { return 4; }
+int sv_abi_version(void) { return 5; }
 
 int sv_device_count(void) {
   int n = 0;
@@ -799,9 +911,7 @@ int sv_batch_set_persistent(void* batch, int blocks_per_cu) {
 }
 
 int sv_poses_destroy(void* poses) {
-  Poses* S = static_cast<Poses*>(poses);
-  if (S) delete S->pose_batch;
-  delete S;
+  delete static_cast<Poses*>(poses);
   return 0;
 }
 
@@ -935,103 +1045,23 @@ int sv_field_census(void* plan, void* poses, unsigned long long* pairs) {
   return rc;
 }
 
-int sv_pose_list_create(void* plan, int n_poses, const double* centers_cart, const double* axes,
-                        void** poses) {
-  if (!plan || !poses || (n_poses > 0 && (!centers_cart || !axes))) return fail(1, "null argument");
-  if (n_poses < 1) return fail(1, "at least one pose required");
-  Plan* P = static_cast<Plan*>(plan);
-  std::vector<double> frac(3 * (size_t)n_poses);
-  for (int i = 0; i < n_poses; ++i) {
-    const double* c = centers_cart + 3 * (size_t)i;
-    const double* u = axes + 3 * (size_t)i;
-    const double nn = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    if (!(std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2])))
-      return fail(1, "nonfinite pose center");
-    if (!std::isfinite(nn) || nn <= 0.0) return fail(1, "finite nonzero directors required");
-    for (int k = 0; k < 3; ++k)
-      frac[3 * (size_t)i + k] = c[0] * P->Hinv[k] + c[1] * P->Hinv[3 + k] + c[2] * P->Hinv[6 + k];
-  }
-  const double unit[3] = {1.0, 0.0, 0.0};
-  Poses* S = nullptr;
-  int rc = build_poses(P, n_poses, frac.data(), 1, unit, 0, 64, &S);
-  if (rc) return rc;
-  S->per_pose = true;
-  S->pose_axes.resize(3 * (size_t)n_poses);
-  std::vector<float> a32(3 * (size_t)n_poses);
-  for (int slot = 0; slot < n_poses; ++slot)
-    for (int k = 0; k < 3; ++k) {
-      const double v = axes[3 * (size_t)S->center_id[slot] + k];
-      S->pose_axes[3 * (size_t)slot + k] = v;
-      a32[3 * (size_t)slot + k] = (float)v;
-    }
-  hipError_t e;
-  if ((e = S->d_pose_axes64.upload(S->pose_axes.data(), S->pose_axes.size(), P->stream)) != hipSuccess ||
-      (e = S->d_pose_axes32.upload(a32.data(), a32.size(), P->stream)) != hipSuccess ||
-      (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
-    delete S;
-    return hip_fail(e, "pose upload");
-  }
-  *poses = S;
-  return 0;
-}
-
-int sv_pose_list_eval(void* poses, int precision, const double* grad_out, double* energy,
-                      double* grad_center, double* grad_axis) {
-  if (!poses) return fail(1, "null pose list");
-  Poses* S = static_cast<Poses*>(poses);
-  if (!S->per_pose) return fail(1, "pose set was not created by sv_pose_list_create");
-  if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
-  Plan* P = S->plan;
-  SV_HIP(hipSetDevice(P->device));
-  if (S->pose_batch && S->pose_precision != precision) {
-    delete S->pose_batch;
-    S->pose_batch = nullptr;
-  }
-  if (!S->pose_batch) {
-    Batch* B = nullptr;
-    Plan* pl = P;
-    Poses* po = S;
-    int rc = batch_build(1, &pl, &po, precision, nullptr, nullptr, nullptr, nullptr, nullptr,
-                         nullptr, &B);
-    if (rc) return rc;
-    S->pose_batch = B;
-    S->pose_precision = precision;
-  }
-  Batch* B = S->pose_batch;
-  Job& J = B->jobs[0];
-  J.grad_out = grad_out;
-  J.pose_e = energy;
-  J.grad_c = grad_center;
-  J.grad_u = grad_axis;
-  SV_HIP(hipMemcpyAsync(B->d_jobs.p, &J, sizeof(Job), hipMemcpyHostToDevice, B->stream));
-  if (!B->lists_built) {
-    int rc = batch_lists(B, false);
-    if (rc) return rc;
-  }
-  SV_HIP(launch_zero(B->d_bad.p, 1, B->stream));
-  const bool grad = grad_center || grad_axis;
-  SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)B->chunks.size(), nullptr,
-                     B->d_offsets.p, B->d_list32.p, B->d_list64.p, B->stream));
-  int bad = 0;
-  SV_HIP(hipMemcpyAsync(&bad, B->d_bad.p, sizeof(int), hipMemcpyDeviceToHost, B->stream));
-  SV_HIP(hipStreamSynchronize(B->stream));
-  if (bad) return fail(4, "nonfinite pose energy or gradient");
-  return 0;
-}
-
 // ---------------------------------------------------------------------------
-// Device-resident pose grids (PoseGrid in sv_objects.h).
+// Device-resident pose grids (PoseGrid in sv_objects.h) and host pose lists.
 // ---------------------------------------------------------------------------
 int sv_pose_grid_create(void* plan, int n_expected, int tile_centers, void** grid) {
   if (!plan || !grid) return fail(1, "null argument");
   if (n_expected < 1) return fail(1, "n_expected must be >= 1");
-  if (tile_centers == 0) tile_centers = 64;
-  if (tile_centers < 1 || tile_centers > 1 << 20) return fail(1, "tile_centers must be in 1..2^20");
+  if (tile_centers < 0 || tile_centers > 1 << 20) return fail(1, "tile_centers must be in 0..2^20");
   Plan* P = static_cast<Plan*>(plan);
   int m[3];
-  choose_tiles(*P, n_expected, 0, tile_centers, m);
+  // bins of about tile_centers site positions (two per pose), or the
+  // automatic choice
+  if (tile_centers > 0)
+    choose_tiles(*P, 2LL * n_expected, 0, tile_centers, m);
+  else
+    choose_site_bins(*P, 2LL * n_expected, m);
   PoseGrid* G = nullptr;
-  const int rc = pose_grid_build(P, m, &G);
+  const int rc = pose_grid_build(P, m, true, &G);
   if (rc) return rc;
   *grid = G;
   return 0;
@@ -1040,7 +1070,31 @@ int sv_pose_grid_create(void* plan, int n_expected, int tile_centers, void** gri
 }  // extern "C"
 
 namespace sv {
+
+Poses::~Poses() { delete list_grid; }
+
 namespace {
+
+// Enqueue the site pass and the pose sums on stream st (events around them
+// when the grid times its phases); the sites of the poses are already binned.
+int site_pass(PoseGrid* G, hipStream_t st, int precision, const double* grad_out, double* energy,
+              double* grad_center, double* grad_axis) {
+  Batch* B = G->B;
+  Poses* S = G->S;
+  const int n = G->binned_n;
+  const long long max_runs = site_runs_bound(G->n_bins, 2LL * n);
+  const bool grad = grad_center || grad_axis;
+  if (G->timing) SV_HIP(hipEventRecord(G->ev[1], st));
+  const SiteArrays a = {S->d_center_rel.p, S->d_center_id.p, G->site_e.p, G->site_g.p,
+                        2LL * G->capacity, G->parts};
+  SV_HIP(launch_site(precision, energy != nullptr, grad, B->d_jobs.p, G->runs.p, (int)max_runs,
+                     G->n_runs.p, B->d_offsets.p, B->d_list32.p, B->d_list64.p, a, st));
+  if (G->timing) SV_HIP(hipEventRecord(G->ev[2], st));
+  SV_HIP(launch_site_combine(n, a, G->P->o0, G->P->o1, grad_out, energy, grad_center, grad_axis,
+                             B->d_bad.p, st));
+  if (G->timing) SV_HIP(hipEventRecord(G->ev[3], st));
+  return 0;
+}
 
 // Enqueue one evaluation of a pose grid on stream st; no host synchronization.
 // Flags of invalid input (2) and nonfinite results (1) accumulate in the
@@ -1049,50 +1103,47 @@ int pose_grid_enqueue(PoseGrid* G, hipStream_t st, int n, const double* centers_
                       const double* axes, int precision, const double* grad_out, double* energy,
                       double* grad_center, double* grad_axis) {
   if (n < 1) return fail(1, "at least one pose required");
+  if (n > (1 << 29)) return fail(1, "at most 2^29 poses per call");
   if (!centers_cart || !axes) return fail(1, "null pose arrays");
   if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
   Batch* B = G->B;
   Poses* S = G->S;
+  Plan* P = G->P;
   G->binned_n = 0;
-  SV_HIP(hipSetDevice(G->P->device));
-  const long long max_chunks = (long long)G->n_bins + (n + 63) / 64;
-  if (max_chunks > INT32_MAX) return fail(1, "too many poses");
+  SV_HIP(hipSetDevice(P->device));
+  const long long n_sites = 2LL * n;
+  const long long max_runs = site_runs_bound(G->n_bins, n_sites);
+  if (max_runs * kMaxSiteParts > INT32_MAX) return fail(1, "too many poses");
   // Work of earlier calls may still use the working buffers: it is ordered on
   // its own stream, which is drained before a call on another stream or
   // before the buffers are reallocated.
   if (G->last_stream_set && G->last_stream != st) SV_HIP(hipStreamSynchronize(G->last_stream));
-  if (n > G->capacity || (size_t)max_chunks > B->d_chunks.cap) {
+  if (n > G->capacity || (size_t)max_runs > G->runs.cap) {
     SV_HIP(hipStreamSynchronize(st));
     if (n > G->capacity) {
-      const size_t c = (size_t)n + (size_t)n / 4;
-      SV_HIP(S->d_center_rel.reserve(3 * c));
-      SV_HIP(S->d_center_id.reserve(c));
-      SV_HIP(S->d_tile_of.reserve(c));
-      SV_HIP(S->d_pose_axes64.reserve(3 * c));
-      SV_HIP(S->d_pose_axes32.reserve(3 * c));
-      SV_HIP(G->bin.reserve(c));
-      SV_HIP(G->rank.reserve(c));
-      G->capacity = (int)std::min<size_t>(c, INT32_MAX);
+      const size_t c = std::min<size_t>((size_t)n + (size_t)n / 4, 1u << 29);
+      SV_HIP(S->d_center_rel.reserve(6 * c));
+      SV_HIP(S->d_center_id.reserve(2 * c));
+      SV_HIP(G->bin.reserve(2 * c));
+      SV_HIP(G->rank.reserve(2 * c));
+      SV_HIP(G->site_e.reserve(2 * c * kMaxSiteParts));
+      SV_HIP(G->site_g.reserve(6 * c * kMaxSiteParts));
+      G->capacity = (int)c;
     }
-    SV_HIP(B->d_chunks.reserve((size_t)max_chunks, true));
+    SV_HIP(G->runs.reserve((size_t)max_runs, true));
   }
   G->last_stream = st;
   G->last_stream_set = true;
-  SV_HIP(launch_job_patch(B->d_jobs.p, S->d_center_rel.p, S->d_center_id.p, S->d_tile_of.p,
-                          S->d_pose_axes64.p, S->d_pose_axes32.p, n, grad_out, energy, grad_center,
-                          grad_axis, G->counts.p, G->n_bins, st));
-  SV_HIP(launch_pose_bin(n, centers_cart, axes, G->Hinv, G->m, G->bin.p, G->rank.p, G->counts.p,
-                         B->d_bad.p, st));
-  SV_HIP(launch_pose_tile_scan(G->counts.p, G->n_bins, 0, G->first.p, B->d_chunks.p,
-                               G->n_chunks.p, st));
-  SV_HIP(launch_pose_scatter(n, centers_cart, axes, G->H, G->Hinv, G->m, G->bin.p, G->rank.p,
-                             G->first.p, S->d_center_rel.p, S->d_center_id.p, S->d_tile_of.p,
-                             S->d_pose_axes64.p, S->d_pose_axes32.p, st));
-  const bool grad = grad_center || grad_axis;
-  SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)max_chunks, G->n_chunks.p,
-                     B->d_offsets.p, B->d_list32.p, B->d_list64.p, st));
+  if (G->timing) SV_HIP(hipEventRecord(G->ev[0], st));
+  SV_HIP(launch_site_bin(n_sites, centers_cart, axes, P->o0, P->o1, G->Hinv, G->m, G->order,
+                         G->bin.p, G->rank.p, G->counts.p, B->d_bad.p, st));
+  SV_HIP(launch_site_scan(G->counts.p, G->n_bins, G->bin_sites.p, G->bin_runs.p, G->first.p,
+                          G->runs.p, G->n_runs.p, st));
+  SV_HIP(launch_site_scatter(n_sites, centers_cart, axes, P->o0, P->o1, G->H, G->Hinv, G->m,
+                             G->bin.p, G->rank.p, G->first.p, G->counts.p, S->d_center_rel.p,
+                             S->d_center_id.p, st));
   G->binned_n = n;
-  return 0;
+  return site_pass(G, st, precision, grad_out, energy, grad_center, grad_axis);
 }
 
 // Evaluates the poses binned by the grid's last enqueue again (new outputs,
@@ -1102,21 +1153,12 @@ int pose_grid_reenqueue(PoseGrid* G, hipStream_t st, int precision, const double
                         double* energy, double* grad_center, double* grad_axis) {
   if (G->binned_n < 1) return fail(1, "no poses binned on this grid yet");
   if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
-  Batch* B = G->B;
-  Poses* S = G->S;
   SV_HIP(hipSetDevice(G->P->device));
   if (G->last_stream_set && G->last_stream != st) SV_HIP(hipStreamSynchronize(G->last_stream));
   G->last_stream = st;
   G->last_stream_set = true;
-  const int n = G->binned_n;
-  const long long max_chunks = (long long)G->n_bins + (n + 63) / 64;
-  SV_HIP(launch_job_patch(B->d_jobs.p, S->d_center_rel.p, S->d_center_id.p, S->d_tile_of.p,
-                          S->d_pose_axes64.p, S->d_pose_axes32.p, n, grad_out, energy, grad_center,
-                          grad_axis, nullptr, 0, st));
-  const bool grad = grad_center || grad_axis;
-  SV_HIP(launch_pose(precision, grad, B->d_jobs.p, B->d_chunks.p, (int)max_chunks, G->n_chunks.p,
-                     B->d_offsets.p, B->d_list32.p, B->d_list64.p, st));
-  return 0;
+  if (G->timing) SV_HIP(hipEventRecord(G->ev[0], st));
+  return site_pass(G, st, precision, grad_out, energy, grad_center, grad_axis);
 }
 
 // Wait for the grid's last stream, report and clear its flags.
@@ -1136,6 +1178,55 @@ int pose_grid_flags(PoseGrid* G) {
 }  // namespace sv
 
 extern "C" {
+
+int sv_pose_list_create(void* plan, int n_poses, const double* centers_cart, const double* axes,
+                        void** poses) {
+  if (!plan || !poses || (n_poses > 0 && (!centers_cart || !axes))) return fail(1, "null argument");
+  if (n_poses < 1) return fail(1, "at least one pose required");
+  Plan* P = static_cast<Plan*>(plan);
+  for (int i = 0; i < n_poses; ++i) {
+    const double* c = centers_cart + 3 * (size_t)i;
+    const double* u = axes + 3 * (size_t)i;
+    const double nn = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    if (!(std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2])))
+      return fail(1, "nonfinite pose center");
+    if (!std::isfinite(nn) || nn <= 0.0) return fail(1, "finite nonzero directors required");
+  }
+  void* grid = nullptr;
+  int rc = sv_pose_grid_create(plan, n_poses, 0, &grid);
+  if (rc) return rc;
+  auto S = new Poses();
+  S->plan = P;
+  S->per_pose = true;
+  S->list_grid = static_cast<PoseGrid*>(grid);
+  S->list_n = n_poses;
+  hipError_t e;
+  if ((e = S->list_centers.upload(centers_cart, 3 * (size_t)n_poses, P->stream)) != hipSuccess ||
+      (e = S->list_axes.upload(axes, 3 * (size_t)n_poses, P->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(P->stream)) != hipSuccess) {
+    delete S;
+    return hip_fail(e, "pose upload");
+  }
+  *poses = S;
+  return 0;
+}
+
+int sv_pose_list_eval(void* poses, int precision, const double* grad_out, double* energy,
+                      double* grad_center, double* grad_axis) {
+  if (!poses) return fail(1, "null pose list");
+  Poses* S = static_cast<Poses*>(poses);
+  if (!S->list_grid) return fail(1, "pose set was not created by sv_pose_list_create");
+  PoseGrid* G = S->list_grid;
+  const hipStream_t st = G->B->stream;
+  // the list's own grid keeps its binned sites between calls
+  const int rc = G->binned_n == S->list_n
+                     ? pose_grid_reenqueue(G, st, precision, grad_out, energy, grad_center,
+                                           grad_axis)
+                     : pose_grid_enqueue(G, st, S->list_n, S->list_centers.p, S->list_axes.p,
+                                         precision, grad_out, energy, grad_center, grad_axis);
+  if (rc) return rc;
+  return pose_grid_flags(G);
+}
 
 int sv_pose_grid_eval(void* grid, int n, const double* centers_cart, const double* axes,
                       int precision, const double* grad_out, double* energy, double* grad_center,
@@ -1180,6 +1271,60 @@ int sv_pose_grid_info(void* grid, int* n_tiles, int* capacity) {
   PoseGrid* G = static_cast<PoseGrid*>(grid);
   if (n_tiles) *n_tiles = G->n_bins;
   if (capacity) *capacity = G->capacity;
+  return 0;
+}
+
+int sv_pose_grid_census(void* grid, unsigned long long* pairs) {
+  if (!grid || !pairs) return fail(1, "null argument");
+  PoseGrid* G = static_cast<PoseGrid*>(grid);
+  if (G->binned_n < 1) return fail(1, "no poses binned on this grid yet");
+  Batch* B = G->B;
+  SV_HIP(hipSetDevice(G->P->device));
+  const hipStream_t st = G->last_stream_set ? G->last_stream : B->stream;
+  const long long max_runs = site_runs_bound(G->n_bins, 2LL * G->binned_n);
+  SV_HIP(launch_zero_u64(B->d_pairs.p, 1, st));
+  SV_HIP(launch_site_census(B->d_jobs.p, G->runs.p, (int)max_runs, G->n_runs.p,
+                            B->d_offsets.p, B->d_list64.p, G->S->d_center_rel.p, B->d_pairs.p,
+                            st));
+  unsigned long long total = 0;
+  SV_HIP(hipMemcpyAsync(&total, B->d_pairs.p, sizeof(total), hipMemcpyDeviceToHost, st));
+  SV_HIP(hipStreamSynchronize(st));
+  *pairs = total;
+  return 0;
+}
+
+int sv_pose_grid_set_parts(void* grid, int parts) {
+  if (!grid) return fail(1, "null pose grid");
+  if (parts < 1 || parts > kMaxSiteParts) return fail(1, "parts must be in 1..8");
+  PoseGrid* G = static_cast<PoseGrid*>(grid);
+  // earlier evaluations read the old layout of the per-site results
+  if (G->last_stream_set) SV_HIP(hipStreamSynchronize(G->last_stream));
+  G->parts = parts;
+  return 0;
+}
+
+int sv_pose_grid_set_timing(void* grid, int on) {
+  if (!grid) return fail(1, "null pose grid");
+  PoseGrid* G = static_cast<PoseGrid*>(grid);
+  SV_HIP(hipSetDevice(G->P->device));
+  if (on)
+    for (auto& ev : G->ev)
+      if (!ev) SV_HIP(hipEventCreate(&ev));
+  G->timing = on != 0;
+  return 0;
+}
+
+int sv_pose_grid_timings(void* grid, double* ms3) {
+  if (!grid || !ms3) return fail(1, "null argument");
+  PoseGrid* G = static_cast<PoseGrid*>(grid);
+  if (!G->timing || G->binned_n < 1) return fail(1, "no timed evaluation on this grid");
+  SV_HIP(hipSetDevice(G->P->device));
+  SV_HIP(hipEventSynchronize(G->ev[3]));
+  for (int k = 0; k < 3; ++k) {
+    float ms = 0.0f;
+    SV_HIP(hipEventElapsedTime(&ms, G->ev[k], G->ev[k + 1]));
+    ms3[k] = ms;
+  }
   return 0;
 }
 

@@ -560,7 +560,7 @@ int sv_strings_create(int n_strings, void* const* plans, int n_images, const dou
       for (int k = 0; k < 3; ++k)
         m[k] = (int)std::min(256.0, std::max(1.0, std::floor(1.0 / (P->hn[k] * kBinEdge) + 0.5)));
       PoseGrid* g = nullptr;
-      const int rc = pose_grid_build(P, m, &g);
+      const int rc = pose_grid_build(P, m, false, &g);
       if (rc) {
         delete Y;
         return rc;

@@ -281,14 +281,16 @@ class Batch:
 class PoseGrid:
     """Device-resident independent poses (sv_pose_grid_*).
 
-    A fixed tile grid over the cell, sized for about ``n_expected`` poses, whose
-    candidate lists are built once.  ``eval`` takes device pointers (e.g. torch
-    ``data_ptr()``) of FP64 ``centers_cart`` and ``axes`` arrays of shape
-    (n, 3), bins them into the tiles on the GPU and writes the requested
+    A fixed grid of bins over the cell, sized for the two site positions of
+    about ``n_expected`` poses (``tile_centers`` sites per bin, or 0 for the
+    library's choice), whose candidate lists are built once.  ``eval``
+    takes device pointers (e.g. torch ``data_ptr()``) of FP64 ``centers_cart``
+    and ``axes`` arrays of shape (n, 3), bins the poses' sites on the GPU,
+    evaluates every site and sums each pose's two, writing the requested
     device outputs; the poses never visit the host.
     """
 
-    def __init__(self, plan: Plan, n_expected: int, tile_centers: int = 64):
+    def __init__(self, plan: Plan, n_expected: int, tile_centers: int = 0):
         self.lib = plan.lib
         self.plan = plan
         handle = ctypes.c_void_p()
@@ -330,6 +332,28 @@ class PoseGrid:
     def check(self):
         """Wait for the last enqueued evaluation; raise on flagged input/results."""
         _lib.check(self.lib.sv_pose_grid_check(self.handle))
+
+    def census(self) -> int:
+        """Exact in-cutoff (site, image) pairs of the last binned poses."""
+        n = ctypes.c_ulonglong()
+        _lib.check(self.lib.sv_pose_grid_census(self.handle, ctypes.byref(n)))
+        return int(n.value)
+
+    def set_parts(self, parts):
+        """Blocks per run of up to 64 sites (1..8): more spread a call's work over
+        more blocks at a fixed cost per block."""
+        _lib.check(self.lib.sv_pose_grid_set_parts(self.handle, int(parts)))
+
+    def set_timing(self, on=True):
+        """Record HIP events around the phases of every later evaluation."""
+        _lib.check(self.lib.sv_pose_grid_set_timing(self.handle, int(bool(on))))
+
+    def timings(self) -> dict:
+        """Milliseconds of the last timed evaluation's phases (waits for it):
+        binning (0 for a re-evaluation), site kernel, pose sums."""
+        ms = np.zeros(3)
+        _lib.check(self.lib.sv_pose_grid_timings(self.handle, _ptr(ms)))
+        return dict(zip(("binning", "site_kernel", "combine"), ms.tolist()))
 
     def info(self):
         tiles, cap = ctypes.c_int(), ctypes.c_int()

@@ -214,3 +214,18 @@ def test_short_cutoff_near_contact_pairs_reach_the_cutoff(oracle, small_case, pr
         if m["accessible_poses"]:
             assert m["accessible_max_abs_error_K"] < 1e-3
         assert m["all_max_relative_error_floor100K"] < 1e-5
+
+
+@pytest.mark.parametrize("name", ["c2h4", "c2h6"])
+def test_pose_sets_normalize_raw_directors(name):
+    """Pose sets take the golden cases' raw directors (|u| up to sqrt 3) and
+    normalize them like the reference (native.py:34): FP64 energies equal the
+    reference kernel's outputs."""
+    from string_vext_amd.field import Plan
+    case, centers, axes, ref = _golden.load(name)
+    assert np.abs(np.linalg.norm(axes, axis=1) - 1).max() > 0.5
+    plan = Plan.from_case(case)
+    for precision, tol in (("fp64", 1e-12), ("mixed", 1e-5)):
+        out = plan.field(plan.poses(centers, axes), precision=precision)["energy"]
+        assert (np.abs(out - ref) / np.maximum(np.abs(ref), 100.0)).max() < tol
+

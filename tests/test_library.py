@@ -37,7 +37,7 @@ def test_binding_covers_header():
 def test_library_reports_no_device_on_cpu_host():
     from string_vext_amd import _lib
     lib = _lib.load(require_device=False)
-    assert lib.sv_abi_version() == 4
+    assert lib.sv_abi_version() == 5
     if lib.sv_device_count() == 0:
         with pytest.raises(RuntimeError):
             _lib.load(require_device=True)
