@@ -46,6 +46,24 @@ int evaluate_reused_neighbors(int n_centers, int n_axes, int n_images, double cu
                               const double *coeff6, const double *shift, const double *floor2,
                               const int64_t *starts, const int32_t *indices, double *output);
 
+/* Allocation-free form of evaluate_reused_neighbors, with the same         */
+/* arguments, results and return codes.  No counterpart in the reference:   */
+/* it is for callers that cannot allocate device memory (serving loops and   */
+/* harnesses that forbid hipMalloc).  Its device staging -- the inputs, the  */
+/* CSR, the output and a flag, each rounded up to 256 bytes -- lives in a    */
+/* static 64 MiB array per device, held until the process ends; a call whose */
+/* staging would exceed it returns 1 with a message in sv_last_error.  The   */
+/* call makes no hipMalloc/hipFree and creates no stream (it runs on the     */
+/* null stream).  Calls in one process are serialized; the array keeps the   */
+/* last call's data, and every array a call reads is written by that call.   */
+int evaluate_reused_neighbors_noalloc(int n_centers, int n_axes, int n_images, double cutoff2,
+                                      const double *centers, const double *axes,
+                                      const double *images, const double *offsets,
+                                      const double *coeff12, const double *coeff6,
+                                      const double *shift, const double *floor2,
+                                      const int64_t *starts, const int32_t *indices,
+                                      double *output);
+
 /* ------------------------------------------------------------------------ */
 /* Library information                                                       */
 /* ------------------------------------------------------------------------ */

@@ -21,6 +21,7 @@ _lib = None
 # (name, restype, argtypes) of every entry point of include/sv_vext.h
 SIGNATURES = [
     ("evaluate_reused_neighbors", _I, [_I, _I, _I, _D] + [_P] * 11),
+    ("evaluate_reused_neighbors_noalloc", _I, [_I, _I, _I, _D] + [_P] * 11),
     ("sv_last_error", ctypes.c_char_p, []),
     ("sv_abi_version", _I, []),
     ("sv_device_count", _I, []),

@@ -747,6 +747,68 @@ struct LegacyCtx {
 std::mutex g_legacy_mutex;
 LegacyCtx* g_legacy = nullptr;
 
+// The reference kernel's input checks in its loop order (kernel.cpp:11-25):
+// the first center whose starts decrease (2) or whose range holds an index
+// out of [0, n_images) (3) stops the evaluation; the centers before it are
+// computed.  Sets *stop (centers to evaluate) and *code.
+void legacy_checks(int n_centers, int n_images, const int64_t* starts, const int32_t* indices,
+                   int* stop, int* code) {
+  *stop = n_centers;
+  *code = 0;
+  for (int c = 0; c < n_centers; ++c) {
+    if (starts[c] > starts[c + 1] || starts[c] < 0) {
+      *stop = c, *code = 2;
+      return;
+    }
+    for (int64_t n = starts[c]; n < starts[c + 1]; ++n)
+      if (indices[n] < 0 || indices[n] >= n_images) {
+        *stop = c, *code = 3;
+        return;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free drop-in entry point: one copy per device,
+// held until the process ends.  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+unsigned char* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// Byte offsets of the drop-in call's arrays in the scratch, each 256-aligned.
+struct ScratchLayout {
+  size_t centers, axes, images, offsets, c12, c6, shift, floor2, starts, indices, output, bad,
+      total;
+};
+
+ScratchLayout scratch_layout(int n_centers, int n_axes, int n_images, int64_t nnz) {
+  ScratchLayout L;
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    const size_t here = at;
+    at += (bytes + 255) / 256 * 256;
+    return here;
+  };
+  L.centers = take(3 * sizeof(double) * (size_t)n_centers);
+  L.axes = take(3 * sizeof(double) * (size_t)n_axes);
+  L.images = take(3 * sizeof(double) * (size_t)n_images);
+  L.offsets = take(2 * sizeof(double));
+  L.c12 = take(sizeof(double) * (size_t)n_images);
+  L.c6 = take(sizeof(double) * (size_t)n_images);
+  L.shift = take(sizeof(double) * (size_t)n_images);
+  L.floor2 = take(sizeof(double) * (size_t)n_images);
+  L.starts = take(sizeof(int64_t) * ((size_t)n_centers + 1));
+  L.indices = take(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1));
+  L.output = take(sizeof(double) * (size_t)n_centers * (size_t)n_axes);
+  L.bad = take(sizeof(int));
+  L.total = at;
+  return L;
+}
+
 }  // namespace
 }  // namespace sv
 
@@ -771,26 +833,8 @@ int evaluate_reused_neighbors(int n_centers, int n_axes, int n_images, double cu
                               const int32_t* indices, double* output) {
   if (n_centers < 0 || n_axes < 1 || n_images < 1 || cutoff2 <= 0) return 1;
   if (n_centers == 0) return 0;
-  // Host validation in the reference's loop order: the first center whose
-  // starts decrease (2) or whose range holds an out-of-range index (3) stops
-  // the evaluation; earlier centers are computed.
   int stop = n_centers, stop_code = 0;
-  for (int c = 0; c < n_centers; ++c) {
-    if (starts[c] > starts[c + 1] || starts[c] < 0) {
-      stop = c, stop_code = 2;
-      break;
-    }
-    bool bad_index = false;
-    for (int64_t n = starts[c]; n < starts[c + 1]; ++n)
-      if (indices[n] < 0 || indices[n] >= n_images) {
-        bad_index = true;
-        break;
-      }
-    if (bad_index) {
-      stop = c, stop_code = 3;
-      break;
-    }
-  }
+  legacy_checks(n_centers, n_images, starts, indices, &stop, &stop_code);
   if (stop == 0) return stop_code;
   std::lock_guard<std::mutex> lock(g_legacy_mutex);
   if (!g_legacy) g_legacy = new LegacyCtx();
@@ -826,6 +870,80 @@ int evaluate_reused_neighbors(int n_centers, int n_axes, int n_images, double cu
   SV_HIP(hipMemcpyAsync(output, L.output.p, (size_t)stop * n_axes * sizeof(double),
                         hipMemcpyDeviceToHost, s));
   SV_HIP(hipMemcpyAsync(&bad_center, L.bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  SV_HIP(hipStreamSynchronize(s));
+  if (bad_center < stop) return 4;
+  return stop_code;
+}
+
+int evaluate_reused_neighbors_noalloc(int n_centers, int n_axes, int n_images, double cutoff2,
+                                      const double* centers, const double* axes,
+                                      const double* images, const double* offsets,
+                                      const double* coeff12, const double* coeff6,
+                                      const double* shift, const double* floor2,
+                                      const int64_t* starts, const int32_t* indices,
+                                      double* output) {
+  if (n_centers < 0 || n_axes < 1 || n_images < 1 || cutoff2 <= 0) return 1;
+  if (n_centers == 0) return 0;
+  int stop = n_centers, stop_code = 0;
+  legacy_checks(n_centers, n_images, starts, indices, &stop, &stop_code);
+  if (stop == 0) return stop_code;
+  const ScratchLayout L = scratch_layout(stop, n_axes, n_images, starts[stop]);
+  if (L.total > kScratchBytes)
+    return fail(1, "evaluate_reused_neighbors_noalloc: " + std::to_string(L.total) +
+                       " bytes of device staging exceed its " +
+                       std::to_string(kScratchBytes >> 20) + " MiB scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(5, "no HIP device");
+  if (dev < 0 || dev >= kMaxDevices) return fail(5, "device ordinal beyond the scratch table");
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    SV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<unsigned char*>(p);
+  }
+  unsigned char* base = g_scratch[dev];
+  auto at = [&](size_t off) { return base + off; };
+  const hipStream_t s = nullptr;   // the device's null stream: no stream is created
+  const int64_t nnz = starts[stop];
+  SV_HIP(hipMemcpyAsync(at(L.centers), centers, 3 * sizeof(double) * (size_t)stop,
+                        hipMemcpyHostToDevice, s));
+  SV_HIP(hipMemcpyAsync(at(L.axes), axes, 3 * sizeof(double) * (size_t)n_axes,
+                        hipMemcpyHostToDevice, s));
+  SV_HIP(hipMemcpyAsync(at(L.images), images, 3 * sizeof(double) * (size_t)n_images,
+                        hipMemcpyHostToDevice, s));
+  SV_HIP(hipMemcpyAsync(at(L.offsets), offsets, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+  SV_HIP(hipMemcpyAsync(at(L.c12), coeff12, sizeof(double) * (size_t)n_images,
+                        hipMemcpyHostToDevice, s));
+  SV_HIP(hipMemcpyAsync(at(L.c6), coeff6, sizeof(double) * (size_t)n_images,
+                        hipMemcpyHostToDevice, s));
+  SV_HIP(hipMemcpyAsync(at(L.shift), shift, sizeof(double) * (size_t)n_images,
+                        hipMemcpyHostToDevice, s));
+  SV_HIP(hipMemcpyAsync(at(L.floor2), floor2, sizeof(double) * (size_t)n_images,
+                        hipMemcpyHostToDevice, s));
+  SV_HIP(hipMemcpyAsync(at(L.starts), starts, sizeof(int64_t) * ((size_t)stop + 1),
+                        hipMemcpyHostToDevice, s));
+  if (nnz > 0)
+    SV_HIP(hipMemcpyAsync(at(L.indices), indices, sizeof(int32_t) * (size_t)nnz,
+                          hipMemcpyHostToDevice, s));
+  // the flag keeps the last call's value: it is set before every launch
+  const int no_bad = 0x7fffffff;
+  SV_HIP(hipMemcpyAsync(at(L.bad), &no_bad, sizeof(int), hipMemcpyHostToDevice, s));
+  SV_HIP(launch_legacy(stop, n_axes, cutoff2, reinterpret_cast<const double*>(at(L.centers)),
+                       reinterpret_cast<const double*>(at(L.axes)),
+                       reinterpret_cast<const double*>(at(L.images)),
+                       reinterpret_cast<const double*>(at(L.offsets)),
+                       reinterpret_cast<const double*>(at(L.c12)),
+                       reinterpret_cast<const double*>(at(L.c6)),
+                       reinterpret_cast<const double*>(at(L.shift)),
+                       reinterpret_cast<const double*>(at(L.floor2)),
+                       reinterpret_cast<const long long*>(at(L.starts)),
+                       reinterpret_cast<const int*>(at(L.indices)),
+                       reinterpret_cast<double*>(at(L.output)),
+                       reinterpret_cast<int*>(at(L.bad)), s));
+  int bad_center = no_bad;
+  SV_HIP(hipMemcpyAsync(output, at(L.output), sizeof(double) * (size_t)stop * n_axes,
+                        hipMemcpyDeviceToHost, s));
+  SV_HIP(hipMemcpyAsync(&bad_center, at(L.bad), sizeof(int), hipMemcpyDeviceToHost, s));
   SV_HIP(hipStreamSynchronize(s));
   if (bad_center < stop) return 4;
   return stop_code;
