@@ -18,7 +18,8 @@
 //      equal Cartesian arc-length fractions, angles unchanged (global.h:1653-1846);
 //   5. angles of images whose energy rose are reverted, angles smoothed with
 //      coefficient 1e-4 and the reference's periodic-unwrap branches
-//      (global.h:1854-2085);
+//      (global.h:1854-2085); in a mixed run the comparison is taken on FP64
+//      energies when the mixed ones are too close to tell;
 //   6. every 200 iterations the string stops once the summed per-image
 //      coordinate and orientation changes fall below 30% of
 //      move x n_images x sqrt(3) (RC_main.cu:1853-1854, 2361-2376);
@@ -29,8 +30,9 @@
 // Energies of string images are evaluated one wave per image: the image is
 // located in a fixed grid of ~2 A bins over the cell whose candidate lists
 // (tile-relative, FP32 + FP64) are built once per framework, and the lanes
-// stride over its bin's list.  One iteration is four kernels; the runtime
-// captures them in a HIP graph and replays it.
+// stride over its bin's list.  One iteration is four kernels (five in a
+// mixed run, with the FP64 recheck); the runtime captures them in a HIP graph
+// and replays it.
 #include "sv_objects.h"
 #include "sv_vext.h"
 
@@ -94,20 +96,54 @@ struct ImageGrid {
 // subtractions are exact or round alike -- at one v_floor_f64.
 __device__ __forceinline__ double wrap01_s(double x) { return x - floor(x); }
 
+// The revert decision of the mixed run (V2 > V0) is taken on FP64 energies
+// whenever the mixed energies differ by no more than their error bound: FP32
+// pair terms are each below the near-contact energy (tens of K), so a sum of
+// ~10^3 of them is off by well under kDecideAbs; the relative part covers the
+// FP64 near-contact terms' share of large energies.
+constexpr double kDecideAbs = 0.05;   // K
+constexpr double kDecideRel = 1e-6;
+
+__device__ __forceinline__ bool decision_in_doubt(double v0, double v2) {
+  const double tol = kDecideAbs + kDecideRel * (fabs(v0) + fabs(v2));
+  return !(fabs(v2 - v0) > tol);   // nonfinite values are in doubt too
+}
+
+// FP64 re-evaluation of the images whose decision is in doubt: waves
+// [0, n_poses) take x0, waves [n_poses, 2 n_poses) take x2; results in Vd.
+struct Recheck {
+  const double* x2;
+  const double* V0;
+  const double* V2;
+};
+
 // Energy (and gradient) of one pose per wave.  x = (a, b, c, alpha, beta,
 // gamma) with fractional center (a, b, c).  The image is located in its bin;
-// lanes stride over the bin's candidate list.  G receives dV/dx.
-template <bool MIXED, bool GRAD>
+// lanes stride over the bin's candidate list.  G receives dV/dx.  RECHECK:
+// the FP64 re-evaluation pass described at Recheck.
+template <bool MIXED, bool GRAD, bool RECHECK = false>
 __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict__ grids,
                                                     const int* __restrict__ plan_of, int n_images,
                                                     int n_poses, const int* __restrict__ converged,
-                                                    const double* __restrict__ x,
-                                                    double* __restrict__ V, double* __restrict__ G) {
+                                                    const double* __restrict__ x_in,
+                                                    double* __restrict__ V, double* __restrict__ G,
+                                                    Recheck rc = Recheck{}) {
   __shared__ int s_near[4][128];   // queued near-contact candidates per wave
   const int lane = threadIdx.x & 63;
   const int wv = uniform_i(threadIdx.x >> 6);
-  const int p = blockIdx.x * 4 + wv;
-  if (p >= n_poses) return;
+  const int slot = blockIdx.x * 4 + wv;   // output index
+  int p = slot;                           // image index
+  const double* __restrict__ x = x_in;
+  if (RECHECK) {
+    if (slot >= 2 * n_poses) return;
+    if (slot >= n_poses) {
+      p = slot - n_poses;
+      x = rc.x2;
+    }
+    if (!decision_in_doubt(rc.V0[p], rc.V2[p])) return;
+  } else if (p >= n_poses) {
+    return;
+  }
   const int s = p / n_images;
   if (converged && converged[s]) return;
   const ImageGrid& IG = grids[plan_of[s]];
@@ -264,7 +300,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
       gu[k] = wave_sum_d(gu[k]);
     }
   if (lane == 0) {
-    V[p] = E;
+    V[slot] = E;
     if (GRAD) {
       // c = f H: dV/df_k = sum_i H[k][i] dV/dc_i
       for (int k = 0; k < 3; ++k)
@@ -389,25 +425,29 @@ __device__ double smooth_one(const double* ang, int i, int N, int k) {
   return (1.0 - kSmooth) * c + 0.5 * kSmooth * (l + r + wraps * 2.0 * kPi);
 }
 
-// Steps 5-7: angle revert, smoothing, convergence test, commit.
+// Steps 5-7: angle revert, smoothing, convergence test, commit.  Vd (mixed
+// runs): FP64 energies at x0 [P] and x2 [P] of the images whose decision
+// was in doubt; null in FP64 runs.
 __global__ __launch_bounds__(256) void string_finish_kernel(StringParams sp,
                                                             int* __restrict__ converged,
                                                             int* __restrict__ iterations,
                                                             double* __restrict__ x0,
                                                             const double* __restrict__ x2,
                                                             const double* __restrict__ V0,
-                                                            const double* __restrict__ V2) {
+                                                            const double* __restrict__ V2,
+                                                            const double* __restrict__ Vd) {
   extern __shared__ double sh[];
   const int s = blockIdx.x;
   if (converged[s]) return;
   const int N = sp.n_images;
+  const int P = sp.n_strings * N;
   double* ang = sh;                 // [N][3] checked s2 angles
   double* red = sh + 3 * N;         // [2 * blockDim]
   const int base = s * N;
   double dc = 0.0, dor = 0.0;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const int p = base + i;
-    const bool revert = V2[p] > V0[p];
+    const bool revert = Vd && decision_in_doubt(V0[p], V2[p]) ? Vd[P + p] > Vd[p] : V2[p] > V0[p];
     double sq = 0.0, so = 0.0;
     for (int k = 0; k < 3; ++k) {
       const double a = revert ? x0[6 * p + 3 + k] : x2[6 * p + 3 + k];
@@ -467,6 +507,16 @@ hipError_t launch_image(int precision, bool grad, const ImageGrid* plans, const 
   return hipGetLastError();
 }
 
+// FP64 energies of the images of a mixed run whose revert decision is in doubt.
+hipError_t launch_recheck(const ImageGrid* plans, const int* plan_of, int N, int P,
+                          const int* converged, const double* x0, const double* x2,
+                          const double* V0, const double* V2, double* Vd, hipStream_t st) {
+  const dim3 g((2 * P + 3) / 4), b(256);
+  hipLaunchKernelGGL((image_kernel<false, false, true>), g, b, 0, st, plans, plan_of, N, P,
+                     converged, x0, Vd, nullptr, Recheck{x2, V0, V2});
+  return hipGetLastError();
+}
+
 }  // namespace
 
 struct StringSys {
@@ -477,7 +527,7 @@ struct StringSys {
   std::vector<PoseGrid*> grids;      // candidate lists per distinct framework
   DevBuf<ImageGrid> d_grids;
   DevBuf<int> d_plan_of, d_dir, d_conv, d_iter;
-  DevBuf<double> d_x0, d_x2, d_G, d_V0, d_V2;
+  DevBuf<double> d_x0, d_x2, d_G, d_V0, d_V2, d_Vd;   // d_Vd: mixed runs' FP64 rechecks [2P]
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~StringSys() {
@@ -508,8 +558,13 @@ int iteration(StringSys* Y) {
   SV_HIP_S(hipGetLastError());
   SV_HIP_S(launch_image(Y->precision, false, Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
                         Y->d_x2.p, Y->d_V2.p, nullptr, st));
+  const bool mixed = Y->precision == PREC_MIXED;
+  if (mixed)
+    SV_HIP_S(launch_recheck(Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p, Y->d_x0.p,
+                            Y->d_x2.p, Y->d_V0.p, Y->d_V2.p, Y->d_Vd.p, st));
   hipLaunchKernelGGL(string_finish_kernel, dim3(Y->S), dim3(256), fin_shm, st, Y->sp, Y->d_conv.p,
-                     Y->d_iter.p, Y->d_x0.p, Y->d_x2.p, Y->d_V0.p, Y->d_V2.p);
+                     Y->d_iter.p, Y->d_x0.p, Y->d_x2.p, Y->d_V0.p, Y->d_V2.p,
+                     mixed ? Y->d_Vd.p : nullptr);
   SV_HIP_S(hipGetLastError());
   return 0;
 }
@@ -593,6 +648,7 @@ int sv_strings_create(int n_strings, void* const* plans, int n_images, const dou
       (e = Y->d_x2.reserve(6 * (size_t)Y->P)) != hipSuccess ||
       (e = Y->d_G.reserve(6 * (size_t)Y->P)) != hipSuccess ||
       (e = Y->d_V0.reserve(Y->P)) != hipSuccess || (e = Y->d_V2.reserve(Y->P)) != hipSuccess ||
+      (precision == PREC_MIXED && (e = Y->d_Vd.reserve(2 * (size_t)Y->P)) != hipSuccess) ||
       (e = hipStreamSynchronize(Y->stream)) != hipSuccess) {
     delete Y;
     return hip_fail(e, "string setup");

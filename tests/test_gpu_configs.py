@@ -12,7 +12,8 @@ field against FP64 on every pose of the 60^3 x 64 grid).
   through torch.autograd, mixed against FP64 at full size and a 2,000-pose
   subset against the oracle's analytic gradient;
 * batched strings: 64 strings x 96 images over 8 full-size and 8 porous
-  frameworks, HIP-graph replay against eager for 50 iterations, and two
+  frameworks, HIP-graph replay against eager for 50 iterations, one mixed
+  iteration against one FP64 iteration from the same state, and two
   iterations of one 96-image string against the CPU restatement.
 Tolerances: FP64 form <= 1e-9 relative (1 K floor; the required bound is
 1e-6); mixed form <= 1e-3 K on accessible poses (E < 2000 K, |E| < 1e4 K) and
@@ -185,3 +186,25 @@ def test_string_iterations_match_cpu_restatement_at_96_images(oracle, string_bat
         np.testing.assert_allclose(path[0, :, 3:], ref[:, 3:], rtol=0, atol=1e-9)
         e_ref = SO.energy_grad(case, ref, setup)[0]
         np.testing.assert_allclose(energy[0], e_ref, rtol=1e-10, atol=1e-7)
+
+
+def test_string_batch_mixed_iteration_shadows_fp64(string_batch):
+    """From the same state (an FP64 run after 200 iterations), one mixed and one
+    FP64 iteration take the same steps and revert decisions: a flipped
+    decision would move an angle by up to the 1 degree step."""
+    from string_vext_amd.strings import StringSystem
+    plans, _, x0, dirs = string_batch
+    base = StringSystem(plans, x0, dirs, precision="fp64")
+    base.run(200, graph=True)
+    state = base.state()[0]
+    step = {}
+    for precision in ("fp64", "mixed"):
+        system = StringSystem(plans, state, dirs, precision=precision)
+        system.run(1, graph=True)
+        step[precision] = system.state()
+    diff = np.abs(step["mixed"][0] - step["fp64"][0])
+    assert diff[..., :3].max() < 1e-7
+    assert diff[..., 3:].max() < 1e-4
+    e64, emx = step["fp64"][1], step["mixed"][1]
+    acc = e64 < 2000.0
+    assert acc.any() and np.abs(emx - e64)[acc].max() < 1e-3

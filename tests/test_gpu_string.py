@@ -57,24 +57,36 @@ def test_graph_replay_equals_eager(init):
 
 
 def test_mixed_profile_close_to_fp64(oracle, init):
-    from string_vext_amd.strings import StringSystem, barrier
-    plan, x0, dirs = init
-    runs = {}
-    for precision in ("fp64", "mixed"):
-        system = StringSystem(plan, x0, dirs, precision=precision)
-        system.run(300, graph=True)
-        runs[precision] = system.state()
-    p64, e64 = runs["fp64"][0], runs["fp64"][1]
-    pmx, emx = runs["mixed"][0], runs["mixed"][1]
-    # same path evaluated in mixed precision: the mixed-energy tolerance
+    """Mixed against FP64 where the comparison is well posed.
+
+    Independent mixed and FP64 runs are not compared: the iteration (fixed-length
+    normalized steps, angle reverts, convergence checks) amplifies rounding-level
+    differences about 1000-fold every 25 iterations, so two FP64 runs whose
+    starting paths differ by 1e-15 end with barriers tens of K apart.  What
+    precision decides is (a) the energy of a given path and (b) one iteration
+    from a given state: same steps, same revert decisions."""
     from string_vext_amd.field import PoseList
-    from string_vext_amd.strings import director
+    from string_vext_amd.strings import StringSystem, barrier, director
+    plan, x0, dirs = init
+    base = StringSystem(plan, x0, dirs, precision="fp64")
+    base.run(300, graph=True)
+    p64, e64 = base.state()[:2]
+    # (a) the FP64 run's path evaluated in mixed precision
     flat = p64.reshape(-1, 6)
     e_same = PoseList(plan, flat[:, :3] @ plan.cell, director(flat[:, 4], flat[:, 5])).energy_host("mixed")
     m = oracle.error_metrics(e64.ravel(), e_same)
-    if m["accessible_poses"]:
-        assert m["accessible_max_abs_error_K"] < 1e-3
-    # independent runs: profiles and barriers agree to 0.01 K on accessible paths
-    ok = (e64.max(axis=1) < 2000.0)
+    assert m["accessible_poses"] > 0
+    assert m["accessible_max_abs_error_K"] < 1e-3
+    ok = e64.max(axis=1) < 2000.0                 # strings accessible end to end
     if ok.any():
-        assert np.abs(barrier(e64[ok]) - barrier(emx[ok])).max() < 1e-2
+        assert np.abs(barrier(e64[ok]) - barrier(e_same.reshape(e64.shape)[ok])).max() < 1e-2
+    # (b) one iteration from the same state in each precision: a flipped revert
+    # decision would move an angle by up to the 1 degree step
+    step = {}
+    for precision in ("fp64", "mixed"):
+        system = StringSystem(plan, p64, dirs, precision=precision)
+        system.run(1, graph=True)
+        step[precision] = system.state()
+    diff = np.abs(step["mixed"][0] - step["fp64"][0])
+    assert diff[..., :3].max() < 1e-7
+    assert diff[..., 3:].max() < 1e-4
