@@ -16,6 +16,13 @@ against it, the field kernel's fraction of the FP32 vector peak (HIP events
 around the kernel on its own stream), and the reference CPU kernel
 (kernel.cpp compiled from source, oracle/_ref) on a bounded sample of the
 same workload on this host's cores.
+
+The default line also carries summaries of the other workloads, each with its
+pair census, time per step, roofline fraction and error against FP64:
+"ragged" (100 frameworks x 2 guests in one launch), "poses" (8 x 65,536
+differentiable poses, forward + backward site kernels) and "strings" (64
+strings x 96 images, 2000 HIP-graph iterations).  --workload NAME prints one
+workload's full line instead.
 """
 from __future__ import annotations
 
@@ -33,6 +40,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "pair-evals/s, 60³×64 Vext, mixed-FP32 vs FP64 GPU at ≤1e-3 K max error"
 FLOP_PER_PAIR = 18          # forward pair-eval, FMA = 2, max/select/rcp/add = 1
+FLOP_PER_GRAD_PAIR = 28     # forward terms + dphi/d(d^2) and its 6 gradient FMAs
 PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 vector (AMD published)
 PEAK_FP64_TFLOPS = 78.6
 SIZE, ORIENTATIONS = 60, 64
@@ -211,17 +219,18 @@ def gpu_error_metrics(ref, cand):
             "all_max_relative_error_floor1K": float(rel.max())}
 
 
-def workload_ragged(args):
+def ragged_line(args):
     """100 frameworks x 2 guests, 60^3 x 64 fields each, one persistent launch."""
     import torch
     from string_vext_amd import autotune, synthetic
     from string_vext_amd.field import Batch, Plan
+    from string_vext_amd.orientation import regular_pose_grid
     t0 = time.perf_counter()
     cases = synthetic.ragged_cases()
     tuned = autotune.tune(cases)
     log(f"autotune: {tuned['tile_centers']} blocks/CU {tuned['blocks_per_cu']} "
         f"({time.perf_counter() - t0:.1f} s)")
-    _, axes = __import__("string_vext_amd.orientation", fromlist=["x"]).regular_pose_grid(SIZE, ORIENTATIONS)
+    _, axes = regular_pose_grid(SIZE, ORIENTATIONS)
     cost = [len(c["frame_frac"]) / np.linalg.det(c["cell"]) * (c["cutoff"] + c["bond_length"] / 2) ** 3
             for c in cases]
     order = np.argsort(cost)[::-1]                  # longest jobs first
@@ -268,7 +277,8 @@ def workload_ragged(args):
                 per_field.append(m)
             del outs
     acc = [m["accessible_max_abs_error_K"] for m in per_field if m["accessible_max_abs_error_K"] is not None]
-    line = {"metric": "pair-evals/s, ragged batch of 100 frameworks x 2 guests, 60^3x64 fields, mixed",
+    achieved = pairs * FLOP_PER_PAIR / (phases["field"] * 1e-3) / 1e12
+    return {"metric": "pair-evals/s, ragged batch of 100 frameworks x 2 guests, 60^3x64 fields, mixed",
             "value": pairs / step_s, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": step_s * 1e3, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -276,30 +286,35 @@ def workload_ragged(args):
                                    "10-60 A, 60-120 deg) x 2 guests, 60^3 x 64 each, one persistent launch",
                        "fields": len(jobs), "pair_evals": pairs, "autotune": tuned["tile_centers"],
                        "blocks_per_cu": tuned["blocks_per_cu"]},
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                         "kernel": "field_mixed_kernel (all fields, one launch)",
+                         "kernel_ms": phases["field"], "flop_per_pair": FLOP_PER_PAIR},
             "phases_ms": phases,
             "error_vs_fp64": {"fields_checked": len(per_field),
                               "max_accessible_max_abs_error_K": max(acc) if acc else None,
                               "fields_with_accessible_poses": len(acc),
                               "max_relative_error_floor1K": max((m["all_max_relative_error_floor1K"] for m in per_field), default=None)},
             "per_field": per_field}
-    print(json.dumps(line), flush=True)
 
 
-def workload_poses(args):
+def poses_line(args):
     """8 full-size frameworks x 65,536 random poses: forward + backward through
-    torch.autograd, mixed and FP64, checked against FP64 and central differences."""
+    torch.autograd, mixed and FP64, checked against FP64 and central differences;
+    then one timed forward + backward per framework for the site kernels' roofline."""
     import torch
     from string_vext_amd import synthetic
-    from string_vext_amd.autograd import pose_energy
+    from string_vext_amd.autograd import _grid_for, pose_energy
     from string_vext_amd.field import Plan
     rng = np.random.default_rng(20261105)
+    n_poses = 65536
     results = {}
     data = []
     for k in range(8):
         case = synthetic.full_field_case(seed=synthetic.FULL_SEED + k)
         plan = Plan.from_case(case)
-        frac = rng.random((65536, 3))
-        u = rng.normal(size=(65536, 3))
+        frac = rng.random((n_poses, 3))
+        u = rng.normal(size=(n_poses, 3))
         u /= np.linalg.norm(u, axis=1, keepdims=True)
         c = torch.tensor(frac @ case["cell"], device="cuda")
         data.append((plan, c, torch.tensor(u, device="cuda")))
@@ -320,6 +335,28 @@ def workload_poses(args):
                 times.append(time.perf_counter() - t)
             outs = step_out
         results[prec] = (float(np.mean(times)), outs)
+    # site-kernel durations (HIP events on torch's stream) and the pair census
+    kernels = {prec: {"binning": 0.0, "forward": 0.0, "backward": 0.0, "combine": 0.0}
+               for prec in ("mixed", "fp64")}
+    pairs = 0
+    for prec in ("mixed", "fp64"):
+        for plan, c, u in data:
+            grid = _grid_for(plan, len(c))
+            grid.set_timing(True)
+            cc = c.clone().requires_grad_(True)
+            uu = u.clone().requires_grad_(True)
+            e = pose_energy(plan, cc, uu, prec)
+            fwd = grid.timings()
+            if prec == "mixed":
+                pairs += grid.census()
+            e.sum().backward()
+            bwd = grid.timings()
+            grid.set_timing(False)
+            kt = kernels[prec]
+            kt["binning"] += fwd["binning"]
+            kt["forward"] += fwd["site_kernel"]
+            kt["backward"] += bwd["site_kernel"]
+            kt["combine"] += fwd["combine"] + bwd["combine"]
     err_e, err_g = [], []
     for (e64, gc64, gu64), (emx, gcmx, gumx) in zip(results["fp64"][1], results["mixed"][1]):
         err_e.append(gpu_error_metrics(e64, emx))
@@ -339,13 +376,27 @@ def workload_poses(args):
         fd = (ep - em) / (2 * h)
         an = results["fp64"][1][0][1][idx, k]
         fd_err = max(fd_err, float(((fd - an).abs() / an.abs().clamp(min=1.0)).max()))
-    n = 8 * 65536
-    line = {"metric": "poses/s, energy + dE/d(center, director) via torch.autograd, 8 x 65,536 poses",
+    n = 8 * n_poses
+    mk = kernels["mixed"]
+    fwd_tf = pairs * FLOP_PER_PAIR / (mk["forward"] * 1e-3) / 1e12
+    bwd_tf = pairs * FLOP_PER_GRAD_PAIR / (mk["backward"] * 1e-3) / 1e12
+    return {"metric": "poses/s, energy + dE/d(center, director) via torch.autograd, 8 x 65,536 poses",
             "value": n / results["mixed"][0], "unit": "poses/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": results["mixed"][0] * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic", "config": {"workload": "8 full-size frameworks (1200 atoms, 37x37x14 A) "
-                                                         "x 65,536 uniform random poses, forward + backward"},
+                                                         "x 65,536 uniform random poses, forward + backward",
+                                            "pair_evals": pairs, "pairs_per_pose": pairs / n},
+            "roofline": {"bound": "valu", "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "forward": {"kernel": "site_kernel<mixed, energy>", "achieved": fwd_tf,
+                                     "frac": fwd_tf / PEAK_FP32_TFLOPS, "flop_per_pair": FLOP_PER_PAIR,
+                                     "kernel_ms_per_framework": mk["forward"] / len(data)},
+                         "backward": {"kernel": "site_kernel<mixed, gradient>", "achieved": bwd_tf,
+                                      "frac": bwd_tf / PEAK_FP32_TFLOPS,
+                                      "flop_per_pair": FLOP_PER_GRAD_PAIR,
+                                      "kernel_ms_per_framework": mk["backward"] / len(data)}},
+            "phases_ms_per_framework": {prec: {k: v / len(data) for k, v in kt.items()}
+                                        for prec, kt in kernels.items()},
             "fp64": {"value": n / results["fp64"][0], "ms_per_step": results["fp64"][0] * 1e3},
             "speedup_mixed_vs_fp64": results["fp64"][0] / results["mixed"][0],
             "energy_error_vs_fp64": {
@@ -353,15 +404,36 @@ def workload_poses(args):
                 "max_relative_error_floor1K": max(m["all_max_relative_error_floor1K"] for m in err_e)},
             "gradient_error_vs_fp64_rel": max(err_g),
             "central_difference_rel_error_h1e-4": fd_err}
-    print(json.dumps(line), flush=True)
 
 
-def workload_strings(args):
-    """64 strings x 96 images, 2000 iterations in a HIP graph, mixed vs FP64, on the
-    full-size frameworks and on a porous variant (400 atoms, same cell and types)."""
+def _string_census(plans, path):
+    """In-cutoff (site, image) pairs of one energy pass over the images of `path`."""
+    import torch
+    from string_vext_amd.field import PoseGrid
+    from string_vext_amd.strings import director
+    pairs = 0
+    for plan in dict.fromkeys(plans):
+        mine = np.array([p is plan for p in plans])
+        flat = path[mine].reshape(-1, 6)
+        c = torch.tensor(np.mod(flat[:, :3], 1) @ plan.cell, device="cuda")
+        u = torch.tensor(director(flat[:, 4], flat[:, 5]), device="cuda")
+        e = torch.empty(len(flat), dtype=torch.float64, device="cuda")
+        grid = PoseGrid(plan, len(flat))
+        grid.eval(len(flat), c.data_ptr(), u.data_ptr(), "fp64", energy=e.data_ptr())
+        pairs += grid.census()
+        grid.close()
+    return pairs
+
+
+def strings_line(args):
+    """64 strings x 96 images, `iterations` iterations in a HIP graph, on the
+    full-size frameworks and on a porous variant (400 atoms, same cell and
+    types): FP64 and mixed runs, a second FP64 run from a start moved by 1e-15
+    (the iteration's own sensitivity), the FP64 path re-evaluated in mixed
+    precision, and one iteration of each precision from the FP64 run's state."""
     from string_vext_amd import synthetic
-    from string_vext_amd.field import Plan
-    from string_vext_amd.strings import StringSystem, barrier, straight_strings
+    from string_vext_amd.field import Plan, PoseList
+    from string_vext_amd.strings import StringSystem, barrier, director, straight_strings
     iters = args.iterations
     report = {}
     for label, n_atoms in (("full", 1200), ("porous", 400)):
@@ -374,58 +446,80 @@ def workload_strings(args):
                 dirs.append(d)
                 init.append(straight_strings(plan, d, 1, 96, seed=1000 * k + j)[0])
         init = np.stack(init)
+        dirs = np.array(dirs)
+        moved = init.copy()
+        moved[:, 1:-1, :3] += 1e-15
         out = {}
-        for prec in ("mixed", "fp64"):
-            system = StringSystem(plans, init, np.array(dirs), precision=prec)
+        for name, prec, start in (("fp64", "fp64", init), ("mixed", "mixed", init),
+                                  ("fp64_moved", "fp64", moved)):
+            system = StringSystem(plans, start, dirs, precision=prec)
             system.run(1, graph=True)                 # capture + first iteration
             t = time.perf_counter()
             system.run(iters - 1, graph=True)
             dt = time.perf_counter() - t
-            out[prec] = (dt / (iters - 1), system.state())
-            log(f"strings {label} {prec}: {dt / (iters - 1) * 1e3:.3f} ms/iteration")
+            out[name] = (dt / (iters - 1), system.state())
+            system.close()
+            log(f"strings {label} {name}: {dt / (iters - 1) * 1e3:.3f} ms/iteration")
         p64, e64, it64, c64 = out["fp64"][1]
-        pmx, emx, itmx, cmx = out["mixed"][1]
-        b64, bmx = barrier(e64), barrier(emx)
-        # the FP64 run's final paths re-evaluated with the mixed-precision kernel
-        from string_vext_amd.field import PoseList
-        from string_vext_amd.strings import director
+        emx, cmx = out["mixed"][1][1], out["mixed"][1][3]
+        emv = out["fp64_moved"][1][1]
+        b64 = barrier(e64)
         same = np.empty_like(e64)
         for s_ in range(len(p64)):
-            pl = PoseList(plans[s_], p64[s_, :, :3] @ plans[s_].cell, director(p64[s_, :, 4], p64[s_, :, 5]))
+            pl = PoseList(plans[s_], np.mod(p64[s_, :, :3], 1) @ plans[s_].cell,
+                          director(p64[s_, :, 4], p64[s_, :, 5]))
             same[s_] = pl.energy_host("mixed")
-        acc64 = e64.max(axis=1) < 2000.0
-        acc = (e64.max(axis=1) < 2000.0) & (emx.max(axis=1) < 2000.0)
-        rel = np.abs(b64 - bmx) / np.maximum(np.abs(b64), 1.0)
+            pl.close()
+        shadow = {}
+        for prec in ("fp64", "mixed"):
+            system = StringSystem(plans, p64, dirs, precision=prec)
+            system.run(1, graph=True)
+            shadow[prec] = system.state()
+            system.close()
+        dstep = np.abs(shadow["mixed"][0] - shadow["fp64"][0])
+        acc = e64.max(axis=1) < 2000.0
+        pairs = _string_census(plans, p64)
+
+        def spread(other):
+            d = np.abs(barrier(other) - b64)[acc]
+            return {"max_K": float(d.max()), "median_K": float(np.median(d))} if acc.any() else None
+
+        ms64, msmx = out["fp64"][0] * 1e3, out["mixed"][0] * 1e3
+        # whole-iteration rate: an energy+gradient pass and an energy pass over
+        # every image, at the final state's pair census
+        flop = pairs * (FLOP_PER_GRAD_PAIR + FLOP_PER_PAIR)
         report[label] = {
-            "ms_per_iteration_mixed": out["mixed"][0] * 1e3,
-            "ms_per_iteration_fp64": out["fp64"][0] * 1e3,
+            "ms_per_iteration_fp64": ms64, "ms_per_iteration_mixed": msmx,
+            "pair_evals_per_pass": pairs,
+            "iteration_rate": {"fp64_TFLOP/s": flop / (ms64 * 1e-3) / 1e12,
+                               "frac_of_fp64_peak": flop / (ms64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                               "mixed_TFLOP/s": flop / (msmx * 1e-3) / 1e12},
             "strings_with_accessible_paths": int(acc.sum()),
-            "max_barrier_diff_K_accessible": float(np.abs(b64 - bmx)[acc].max()) if acc.any() else None,
-            "max_profile_diff_K_accessible": float(np.abs(e64 - emx)[acc].max()) if acc.any() else None,
-            "median_relative_barrier_diff_all": float(np.median(rel)),
-            "same_path_strings_accessible": int(acc64.sum()),
-            "same_path_max_profile_diff_K": float(np.abs(same - e64)[acc64].max()) if acc64.any() else None,
-            "same_path_max_barrier_diff_K": float(np.abs(barrier(same) - b64)[acc64].max()) if acc64.any() else None,
-            "iterations_fp64": it64.tolist(), "converged_fp64": int(c64.sum()),
+            "same_path_max_profile_diff_K": float(np.abs(same - e64)[acc].max()) if acc.any() else None,
+            "same_path_max_barrier_diff_K": float(np.abs(barrier(same) - b64)[acc].max()) if acc.any() else None,
+            "one_iteration_max_coord_diff": float(dstep[..., :3].max()),
+            "one_iteration_max_angle_diff_rad": float(dstep[..., 3:].max()),
+            "independent_barrier_spread_mixed_vs_fp64": spread(emx),
+            "independent_barrier_spread_fp64_moved_1e-15_vs_fp64": spread(emv),
+            "iterations_fp64": sorted(set(it64.tolist())), "converged_fp64": int(c64.sum()),
             "converged_mixed": int(cmx.sum()),
-            "barriers_fp64_K": [round(float(v), 6) for v in b64],
-            "barriers_mixed_K": [round(float(v), 6) for v in bmx]}
+            "barriers_fp64_K": [round(float(v), 6) for v in b64[acc]],
+            "barriers_mixed_K": [round(float(v), 6) for v in barrier(emx)[acc]]}
     # The FP64 form is the faster one here (FP64 issue runs at the FP32 rate and
     # the image sweep is short), so it is the reported value; the mixed form's
-    # time and its same-path error are reported beside it.
-    line = {"metric": "string iterations/s, 64 strings x 96 images, hipGraph",
+    # time and its errors are reported beside it.
+    return {"metric": "string iterations/s, 64 strings x 96 images, hipGraph",
             "value": 1e3 / report["full"]["ms_per_iteration_fp64"], "unit": "iterations/s",
             "n_gpus": 1, "steps": iters, "warmup": 1,
             "ms_per_step": report["full"]["ms_per_iteration_fp64"], "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "64 strings x 96 images (8 frameworks x 8 strings, straight "
-                                   "fractional paths along a/b/c), 2000 iterations, one iteration "
-                                   "per HIP graph replay"},
+            "config": {"workload": f"64 strings x 96 images (8 frameworks x 8 strings, straight "
+                                   f"fractional paths along a/b/c), {iters} iterations, one iteration "
+                                   f"per HIP graph replay"},
             "frameworks": report}
-    print(json.dumps(line), flush=True)
 
 
-def workload_dropin(args):
+def dropin_line(args):
     """The reference's own call sequence with this library as its --kernel: the
     full 60^3 x 64 field in 512-center batches, cKDTree CSR on the host, one
     evaluate_reused_neighbors call per batch (FP64, host buffers, tables sent
@@ -452,7 +546,7 @@ def workload_dropin(args):
     ref = plan.field(poses, precision="fp64")["energy"]
     rel = float((np.abs(out - ref) / np.maximum(np.abs(ref), 1.0)).max())
     wall, query, compute = (float(np.mean(v)) for v in (walls, queries, computes))
-    line = {"metric": "pair-evals/s, 60^3 x 64 Vext through the reference's call sequence "
+    return {"metric": "pair-evals/s, 60^3 x 64 Vext through the reference's call sequence "
                       "(evaluate_reused_neighbors per 512-center batch, host cKDTree CSR)",
             "value": pairs / wall, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
             "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -463,42 +557,18 @@ def workload_dropin(args):
             "query_seconds": query, "call_seconds": compute,
             "calls_value": pairs / compute,
             "max_relative_vs_batched_fp64_floor1K": rel}
-    print(json.dumps(line), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", choices=("field", "ragged", "poses", "strings", "dropin"),
-                    default="field",
-                    help="field (default): the 60^3 x 64 headline; the others measure the "
-                         "ragged batch, differentiable poses, batched strings and the drop-in "
-                         "C ABI route")
-    ap.add_argument("--iterations", type=int, default=2000, help="strings workload iterations")
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--fp64-steps", type=int, default=5)
-    ap.add_argument("--cpu-sample", type=int, default=16384,
-                    help="grid centers in the CPU baseline sample (0 skips it)")
-    ap.add_argument("--no-check", action="store_true", help="skip the mixed-vs-FP64 error pass")
-    ap.add_argument("--blocks-per-cu", type=int, default=None,
-                    help="ragged workload: persistent grid size (0 = plain grid; default autotuned)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="also run a plain elementwise copy of MIB MiB (FP32 in, FP32 out) so that "
-                         "PMC byte counters can be scaled against a kernel of known traffic")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): every pose is independent, "
-              "so N GPUs would each evaluate their own slab of centers with no communication; "
-              "this benchmark measures one GPU's share at the full field shape.", file=sys.stderr)
-        sys.exit(2)
-    if args.steps < 1:
-        sys.exit("--steps must be >= 1")
-    if args.workload != "field":
-        {"ragged": workload_ragged, "poses": workload_poses, "strings": workload_strings,
-         "dropin": workload_dropin}[args.workload](args)
-        return
+def compact(line, keep):
+    """The headline line's summary of another workload's line."""
+    out = {k: line[k] for k in ("metric", "value", "unit", "ms_per_step", "steps") if k in line}
+    out.update({k: line[k] for k in keep if k in line})
+    return out
 
+
+def field_line(args):
+    """The headline: the 60^3 x 64 field of the 1200-atom framework, mixed and FP64,
+    with the reference CPU kernel on a sample of the same workload."""
     from string_vext_amd import synthetic
     from string_vext_amd.field import Plan
     from string_vext_amd.orientation import regular_pose_grid
@@ -506,9 +576,12 @@ def main():
     case = synthetic.full_field_case()
     centers, axes = regular_pose_grid(SIZE, ORIENTATIONS)
     from string_vext_amd import autotune
-    tuned = autotune.tune([case], size=SIZE, orientations=ORIENTATIONS, reps=2, persistent=False)
-    tile_centers = tuned["tile_centers"][autotune.density_class(case)]
-    log(f"autotune: tile_centers={tile_centers} trials={tuned['trials']}")
+    if args.tile_centers > 0:
+        tile_centers = args.tile_centers
+    else:
+        tuned = autotune.tune([case], size=SIZE, orientations=ORIENTATIONS, reps=2, persistent=False)
+        tile_centers = tuned["tile_centers"][autotune.density_class(case)]
+        log(f"autotune: tile_centers={tile_centers} trials={tuned['trials']}")
     plan = Plan.from_case(case)
     poses = plan.grid(SIZE, axes, tile_centers=tile_centers)
     pairs = plan.census(poses)
@@ -529,10 +602,12 @@ def main():
         log(f"mixed vs fp64: {errors}")
 
     achieved = pairs * FLOP_PER_PAIR / (field_ms * 1e-3) / 1e12
+    # HBM bytes per launch from PMC counters, profiled at each tile size
     traffic = None
     prof = ROOT / "profiles" / "field_mixed_60x64_traffic.json"
     if prof.exists():
-        traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
+        rec = json.loads(prof.read_text()).get("by_tile", {}).get(str(tile_centers))
+        traffic = rec and rec["hbm_bytes_per_launch"]
 
     cpu = cpu_all = small = None
     if args.cpu_sample > 0:
@@ -568,6 +643,7 @@ def main():
                    "tile_centers": tile_centers},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "traffic_tile_centers": tile_centers if traffic else None,
                      "kernel": "field_mixed_kernel", "kernel_ms": field_ms,
                      "flop_per_pair": FLOP_PER_PAIR},
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
@@ -579,6 +655,63 @@ def main():
         "error_vs_fp64": errors,
         "phases_ms": t_mixed,
     }
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--workload", choices=("field", "ragged", "poses", "strings", "dropin"),
+                    default="field",
+                    help="field (default): the 60^3 x 64 headline, with summaries of the ragged, "
+                         "poses and strings workloads; the others print one workload's full line "
+                         "(dropin: the reference's call sequence through the C ABI)")
+    ap.add_argument("--iterations", type=int, default=2000, help="strings workload iterations")
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=5)
+    ap.add_argument("--cpu-sample", type=int, default=16384,
+                    help="grid centers in the CPU baseline sample (0 skips it)")
+    ap.add_argument("--no-check", action="store_true", help="skip the mixed-vs-FP64 error pass")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="field workload: skip the ragged, poses and strings summaries")
+    ap.add_argument("--tile-centers", type=int, default=0,
+                    help="field workload: centers per tile (0 = autotuned)")
+    ap.add_argument("--blocks-per-cu", type=int, default=None,
+                    help="ragged workload: persistent grid size (0 = plain grid; default autotuned)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain elementwise copy of MIB MiB (FP32 in, FP32 out) so that "
+                         "PMC byte counters can be scaled against a kernel of known traffic")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): every pose is independent, "
+              "so N GPUs would each evaluate their own slab of centers with no communication; "
+              "this benchmark measures one GPU's share at the full field shape.", file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1:
+        sys.exit("--steps must be >= 1")
+    if args.workload != "field":
+        line = {"ragged": ragged_line, "poses": poses_line, "strings": strings_line,
+                "dropin": dropin_line}[args.workload](args)
+        print(json.dumps(line), flush=True)
+        return
+    line = field_line(args)
+    if not args.headline_only:
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = 3, 1
+        line["ragged"] = compact(ragged_line(sub), ("config", "roofline", "error_vs_fp64"))
+        log(f"ragged: {line['ragged']['value']:.3e} pair-evals/s")
+        sub.steps, sub.warmup = 5, 2
+        line["poses"] = compact(poses_line(sub), (
+            "config", "roofline", "phases_ms_per_framework", "fp64", "energy_error_vs_fp64",
+            "gradient_error_vs_fp64_rel", "central_difference_rel_error_h1e-4"))
+        log(f"poses: {line['poses']['value']:.3e} poses/s")
+        sl = strings_line(args)
+        line["strings"] = compact(sl, ("config",))
+        line["strings"]["frameworks"] = {
+            k: {kk: vv for kk, vv in v.items() if not kk.startswith("barriers_")}
+            for k, v in sl["frameworks"].items()}
+        log(f"strings: {line['strings']['value']:.1f} iterations/s")
     print(json.dumps(line), flush=True)
 
 
