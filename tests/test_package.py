@@ -1,5 +1,6 @@
 """The boundary ships as a package whose ``string-vext`` console script (the
-reference's script name, its pyproject.toml [project.scripts]) resolves to this
+reference's script name, its pyproject.toml [project.scripts]; here in
+setup.cfg, which setuptools 59 reads) resolves to this
 package's CLI, which keeps the reference's refusal to overwrite an output."""
 import importlib
 from pathlib import Path
@@ -10,9 +11,12 @@ ROOT = Path(__file__).resolve().parents[1]
 
 
 def _scripts():
-    import tomli
-    with open(ROOT / "pyproject.toml", "rb") as handle:
-        return tomli.load(handle)["project"]["scripts"]
+    """console_scripts of setup.cfg (name = module:attr per line)."""
+    import configparser
+    cfg = configparser.ConfigParser()
+    cfg.read(ROOT / "setup.cfg")
+    lines = cfg["options.entry_points"]["console_scripts"].strip().splitlines()
+    return dict((s.strip() for s in line.split("=", 1)) for line in lines)
 
 
 def test_console_script_resolves_to_the_cli():
