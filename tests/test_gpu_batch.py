@@ -67,3 +67,28 @@ def test_batch_matches_oracle_and_tiles_do_not_change_results(oracle, ragged):
                     m = oracle.error_metrics(ref[j], got)
                     if m["accessible_poses"]:
                         assert m["accessible_max_abs_error_K"] < 1e-3
+
+
+def test_autotune_times_real_batches(ragged, tmp_path, monkeypatch):
+    """The autotuner on the GPU: every candidate tile size and persistent grid
+    is timed (positive finite milliseconds), the choice is a candidate, and the
+    field at the chosen tile size equals the oracle-checked tile-64 field."""
+    from string_vext_amd import autotune
+    from string_vext_amd.field import Batch
+    monkeypatch.setenv("XDG_CACHE_HOME", str(tmp_path))
+    res = autotune.tune(ragged[:3], size=8, orientations=16, reps=1)
+    assert res["blocks_per_cu"] in autotune.BLOCK_CANDIDATES
+    assert set(res["tile_centers"].values()) <= set(autotune.TILE_CANDIDATES)
+    tiles = [t for t in res["trials"] if "tile_centers" in t]
+    assert len(tiles) == len(autotune.TILE_CANDIDATES) * len(res["tile_centers"])
+    assert all(np.isfinite(t["ms"]) and t["ms"] > 0 for t in res["trials"])
+    case = ragged[0]
+    tile = res["tile_centers"][autotune.density_class(case)]
+    out = {}
+    for tc in (64, tile):
+        jobs = _jobs([case], [8], [16], tile_centers=tc)
+        batch = Batch(jobs, precision="fp64", outputs=("energy",))
+        batch.run(sync=True)
+        out[tc] = batch.fetch("energy")
+        batch.close()
+    np.testing.assert_allclose(out[tile], out[64], rtol=1e-12, atol=1e-9)
