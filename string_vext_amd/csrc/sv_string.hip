@@ -109,6 +109,10 @@ __device__ __forceinline__ bool decision_in_doubt(double v0, double v2) {
   return !(fabs(v2 - v0) > tol);   // nonfinite values are in doubt too
 }
 
+// Type count up to which the FP64 image sweep runs as one stream over the
+// bin's type segments (more types: one sweep per type).
+constexpr int kStreamTypes = 16;
+
 // FP64 re-evaluation of the images whose decision is in doubt: waves
 // [0, n_poses) take x0, waves [n_poses, 2 n_poses) take x2; results in Vd.
 struct Recheck {
@@ -129,6 +133,10 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
                                                     double* __restrict__ V, double* __restrict__ G,
                                                     Recheck rc = Recheck{}) {
   __shared__ int s_near[4][128];   // queued near-contact candidates per wave
+  // FP64 sweep: per-type coefficients (c12, c6, shift, floor2) and the bin's
+  // type segment bounds, per wave
+  __shared__ double4 s_coef[4][kStreamTypes];
+  __shared__ long long s_bound[4][kStreamTypes + 1];
   const int lane = threadIdx.x & 63;
   const int wv = uniform_i(threadIdx.x >> 6);
   const int slot = blockIdx.x * 4 + wv;   // output index
@@ -197,7 +205,33 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
       }
     }
   };
-  for (int t = 0; t < J.n_types; ++t) {
+  const int n_types = J.n_types;
+  const bool stream = !MIXED && n_types <= kStreamTypes;
+  if (stream) {
+    // One sweep over the bin's type segments, which are contiguous: each lane
+    // finds its candidate's type from the segment bounds and takes the type's
+    // coefficients from LDS, so the sweep has no break per type and the next
+    // candidate is always in flight.
+    if (lane < n_types)
+      s_coef[wv][lane] = make_double4(J.c12[lane], J.c6[lane], J.shift[lane], J.floor2[lane]);
+    if (lane <= n_types) s_bound[wv][lane] = IG.offsets[seg0 + lane];
+    wave_barrier_s();
+    const long long end = s_bound[wv][n_types];
+    long long i = s_bound[wv][0] + lane;
+    int ty = 0;
+    double q[3] = {0.0, 0.0, 0.0};
+    if (i < end)
+      for (int k = 0; k < 3; ++k) q[k] = IG.list64[3 * i + k];
+    for (; i < end; i += 64) {
+      const double cur[3] = {q[0], q[1], q[2]};
+      if (i + 64 < end)
+        for (int k = 0; k < 3; ++k) q[k] = IG.list64[3 * (i + 64) + k];
+      while (i >= s_bound[wv][ty + 1]) ++ty;   // i < end = s_bound[n_types]: ty < n_types
+      const double4 co = s_coef[wv][ty];
+      pair64r(cur, co.x, co.y, co.z, co.w);
+    }
+  }
+  for (int t = 0; t < n_types && !stream; ++t) {
     const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
     const float near2 = J.near2f[t];
     const long long beg = IG.offsets[seg0 + t], end = IG.offsets[seg0 + t + 1];

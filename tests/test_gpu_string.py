@@ -90,3 +90,37 @@ def test_mixed_profile_close_to_fp64(oracle, init):
     diff = np.abs(step["mixed"][0] - step["fp64"][0])
     assert diff[..., :3].max() < 1e-7
     assert diff[..., 3:].max() < 1e-4
+
+
+@pytest.mark.parametrize("n_types", [1, 60])
+def test_iterations_match_cpu_restatement_for_type_counts(oracle, n_types):
+    """The FP64 image sweep runs as one stream over a bin's type segments up to
+    16 LJ types, and one sweep per type beyond: both against the CPU
+    restatement (60 atoms of one type, or 60 atoms each its own type)."""
+    from oracle import string_oracle as SO
+    from string_vext_amd import synthetic
+    from string_vext_amd.cell import cell_matrix
+    from string_vext_amd.field import Plan
+    from string_vext_amd.strings import StringSystem, straight_strings
+    rng = np.random.default_rng(60 + n_types)
+    lengths, angles = (18.0, 18.0, 12.0), (90.0, 90.0, 120.0)
+    frac = synthetic.random_frame(rng, cell_matrix(lengths, angles), 60, min_dist=2.5)
+    if n_types == 1:
+        sigma, eps = np.full(60, 3.3), np.full(60, 30.0)
+    else:
+        sigma, eps = rng.uniform(2.6, 3.8, 60), rng.uniform(8.0, 60.0, 60)
+    case = synthetic.make_case(lengths, angles, frac, sigma, eps)
+    plan = Plan.from_case(case)
+    assert plan.n_types == n_types
+    x0 = straight_strings(plan, 3, 1, 12, seed=5, scan=6, orientations=4, probes=8)
+    setup = oracle.ReuseSetup(case)
+    system = StringSystem(plan, x0, np.array([3]), precision="fp64")
+    ref = x0[0]
+    for it in range(2):
+        ref = SO.iterate(case, ref, 3, it, setup=setup)[0]
+    system.run(2, graph=True)
+    path, energy, _, _ = system.state()
+    np.testing.assert_allclose(path[0, :, :3], ref[:, :3], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(path[0, :, 3:], ref[:, 3:], rtol=0, atol=1e-9)
+    e_ref = SO.energy_grad(case, ref, setup)[0]
+    np.testing.assert_allclose(energy[0], e_ref, rtol=1e-10, atol=1e-7)
