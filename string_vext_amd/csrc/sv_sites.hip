@@ -96,66 +96,21 @@ struct SitePartial {
   float e, g0, g1, g2;
 };
 
-// The lane's site relative to its run's sphere centre m (units of rc):
-// s' = p - m, and a = -2 s', b = |s'|^2 for the expanded squared distance.
-struct SiteLane {
-  float sx, sy, sz, ax, ay, az, b;
-};
-
-// x = |e' - s'|^2 for a staged candidate (e' = e - m, |e'|^2) expanded as
-// (|e'|^2 + b) + a.e': one add and three fma, against three subtractions, a
-// multiply and two fma for the difference vector.  The terms are no larger
-// than the squared reach of the run's sphere, so a close pair (small |e'|)
-// rounds about as finely as the FP32 list coordinates are stored.  One asm
-// block: left to itself the compiler keeps a copy of a and b per unrolled
-// candidate and doubles the kernel's registers.  CLAMP: the last fma clamps
-// x to [0, 1].
-template <bool CLAMP>
-__device__ __forceinline__ float site_x_expanded(const float4 e, const SiteLane& L) {
-  float x;
-  if (CLAMP)
-    asm("v_add_f32 %0, %1, %2\n\t"
-        "v_fmac_f32 %0, %3, %4\n\t"
-        "v_fmac_f32 %0, %5, %6\n\t"
-        "v_fmac_f32_e64 %0, %7, %8 clamp"
-        : "=&v"(x)
-        : "v"(e.w), "v"(L.b), "v"(e.z), "v"(L.az), "v"(e.y), "v"(L.ay), "v"(e.x), "v"(L.ax));
-  else
-    asm("v_add_f32 %0, %1, %2\n\t"
-        "v_fmac_f32 %0, %3, %4\n\t"
-        "v_fmac_f32 %0, %5, %6\n\t"
-        "v_fmac_f32 %0, %7, %8"
-        : "=&v"(x)
-        : "v"(e.w), "v"(L.b), "v"(e.z), "v"(L.az), "v"(e.y), "v"(L.ay), "v"(e.x), "v"(L.ax));
-  return x;
-}
-
-// One staged candidate against the lane's site (units of rc).  The energy
-// pass takes the expanded distance; the gradient pass needs the difference
-// vector r = e' - s' and forms x from it (the expanded form there costs the
-// gradient kernel a wave per SIMD in registers and gains nothing).
+// One staged candidate against the lane's site p (both in units of rc).
 // NEAR (class 0): a candidate within the near-contact radius (x < nsx) takes
 // the clamped FP32 term and returns true for its FP64 term.
 template <bool NEAR, bool GRAD, bool ENERGY>
-__device__ __forceinline__ bool site_term32(const float4 e, const SiteLane& L, float s6,
-                                            float m2s6, float nsx, SitePartial& a) {
-  float x, rx = 0.f, ry = 0.f, rz = 0.f;
+__device__ __forceinline__ bool site_term32(const float4 e, float pxf, float pyf, float pzf,
+                                            float s6, float m2s6, float nsx, SitePartial& a) {
+  const float rx = e.x - pxf, ry = e.y - pyf, rz = e.z - pzf;
+  float x;
   bool near = false;
-  if (GRAD) {
-    rx = e.x - L.sx, ry = e.y - L.sy, rz = e.z - L.sz;
+  if (NEAR) {
     const float xr = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
-    if (NEAR) {
-      near = xr < nsx;
-      x = near ? 1.0f : fminf(xr, 1.0f);
-    } else {
-      x = __builtin_amdgcn_fmed3f(xr, 0.0f, 1.0f);
-    }
-  } else if (NEAR) {
-    const float xr = site_x_expanded<false>(e, L);
     near = xr < nsx;
     x = near ? 1.0f : fminf(xr, 1.0f);
   } else {
-    x = site_x_expanded<true>(e, L);
+    x = __builtin_amdgcn_fmed3f(fmaf(rz, rz, fmaf(ry, ry, rx * rx)), 0.0f, 1.0f);
   }
   const float i = __builtin_amdgcn_rcpf(x);
   const float i3 = i * i * i;
@@ -184,8 +139,8 @@ struct SiteType {
 template <bool NEAR, bool GRAD, bool ENERGY>
 __device__ __forceinline__ void site_batch32(const float4* __restrict__ s_e,
                                              const double (*__restrict__ s_d)[64], int n,
-                                             const SiteLane& L, double px, double py, double pz,
-                                             double rc2, const SiteType& ty, double& E,
+                                             float pxf, float pyf, float pzf, double px, double py,
+                                             double pz, double rc2, const SiteType& ty, double& E,
                                              double G[3]) {
   SitePartial a = {0.f, 0.f, 0.f, 0.f};
   unsigned long long marks = 0ull;
@@ -194,11 +149,11 @@ __device__ __forceinline__ void site_batch32(const float4* __restrict__ s_e,
     if (q0 + kSiteGroup <= n) {
 #pragma unroll
       for (int j = 0; j < kSiteGroup; ++j)
-        if (site_term32<NEAR, GRAD, ENERGY>(s_e[q0 + j], L, ty.s6, ty.m2s6, ty.nsx, a))
+        if (site_term32<NEAR, GRAD, ENERGY>(s_e[q0 + j], pxf, pyf, pzf, ty.s6, ty.m2s6, ty.nsx, a))
           m16 |= 1u << j;
     } else {
       for (int j = 0; q0 + j < n; ++j)
-        if (site_term32<NEAR, GRAD, ENERGY>(s_e[q0 + j], L, ty.s6, ty.m2s6, ty.nsx, a))
+        if (site_term32<NEAR, GRAD, ENERGY>(s_e[q0 + j], pxf, pyf, pzf, ty.s6, ty.m2s6, ty.nsx, a))
           m16 |= 1u << j;
     }
     if (NEAR) {
@@ -277,16 +232,8 @@ __global__ __launch_bounds__(64 * kSiteWaves) void site_kernel(const Job* __rest
   const float reach = uniform_f(sqrtf(wave_max_f(fmaf(dcz, dcz, fmaf(dcy, dcy, dcx * dcx))))) +
                       1.0f + 1e-4f;
   const float reach2 = run.whole ? reach * reach : INFINITY;
-  SiteLane sl;
-  sl.sx = dcx, sl.sy = dcy, sl.sz = dcz;
-  sl.ax = -2.0f * dcx, sl.ay = -2.0f * dcy, sl.az = -2.0f * dcz;
-  sl.b = fmaf(dcz, dcz, fmaf(dcy, dcy, dcx * dcx));
-  // keep the lane constants the pass uses in registers (left alone, the
-  // compiler forms them again per batch)
-  if (GRAD)
-    asm volatile("" : "+v"(sl.sx), "+v"(sl.sy), "+v"(sl.sz));
-  else
-    asm volatile("" : "+v"(sl.ax), "+v"(sl.ay), "+v"(sl.az), "+v"(sl.b));
+  // keep the FP32 site in registers (left alone, the compiler converts it again per batch)
+  asm volatile("" : "+v"(pxf), "+v"(pyf), "+v"(pzf));
   const double rc2 = J.rc2;
   const long long seg0 = J.seg_base + (long long)run.bin * J.n_types * 2;
   double E = 0.0, G[3] = {0.0, 0.0, 0.0};
@@ -313,12 +260,11 @@ __global__ __launch_bounds__(64 * kSiteWaves) void site_kernel(const Job* __rest
            base += 64LL * dealers) {
         const long long idx = base + lane;
         bool keep = false;
-        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);   // staged as (e - m, |e - m|^2)
+        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
         if (idx < end) {
-          const float4 v = list32[idx];
-          e.x = v.x - cxf, e.y = v.y - cyf, e.z = v.z - czf;
-          e.w = fmaf(e.z, e.z, fmaf(e.y, e.y, e.x * e.x));
-          keep = e.w <= reach2;
+          e = list32[idx];
+          const float ex = e.x - cxf, ey = e.y - cyf, ez = e.z - czf;
+          keep = fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= reach2;
         }
         const unsigned long long kept = __ballot(keep);
         if (keep) {
@@ -335,9 +281,9 @@ __global__ __launch_bounds__(64 * kSiteWaves) void site_kernel(const Job* __rest
         if (MIXED) {
           n_fp += n;
           if (cls == 0)
-            site_batch32<true, GRAD, ENERGY>(s_e, s_d, n, sl, px, py, pz, rc2, ty, E, G);
+            site_batch32<true, GRAD, ENERGY>(s_e, s_d, n, pxf, pyf, pzf, px, py, pz, rc2, ty, E, G);
           else
-            site_batch32<false, GRAD, ENERGY>(s_e, s_d, n, sl, px, py, pz, rc2, ty, E, G);
+            site_batch32<false, GRAD, ENERGY>(s_e, s_d, n, pxf, pyf, pzf, px, py, pz, rc2, ty, E, G);
         } else {
           for (int q = 0; q < n; ++q)
             site_pair64<GRAD>(s_d[0][q] - px, s_d[1][q] - py, s_d[2][q] - pz, rc2, ty.c12, ty.c6,
