@@ -239,3 +239,39 @@ def test_string_batch_mixed_iteration_shadows_fp64(string_batch):
     e64, emx = step["fp64"][1], step["mixed"][1]
     acc = e64 < 2000.0
     assert acc.any() and np.abs(emx - e64)[acc].max() < 1e-3
+
+
+def test_accessible_pore_poses_mixed_against_fp64(oracle):
+    """Independent poses where precision is hardest: accessible poses of the
+    porous 400-atom framework (in pores, against the walls), taken from its
+    60^3 x 64 field, evaluated through the pose grid (site kernels) in both
+    precisions: energies within 1e-3 K, gradients within 1e-4 relative."""
+    import torch
+    from string_vext_amd import synthetic
+    from string_vext_amd.autograd import pose_energy
+    from string_vext_amd.field import Plan
+    from string_vext_amd.orientation import regular_pose_grid
+    case = synthetic.full_field_case(n_atoms=400)
+    plan = Plan.from_case(case)
+    centers, axes = regular_pose_grid(SIZE, K)
+    e_field = plan.field(plan.grid(SIZE, axes), precision="fp64")["energy"]
+    ic, ik = np.nonzero(e_field < 2000.0)
+    pick = np.random.default_rng(3).choice(len(ic), size=min(65536, len(ic)), replace=False)
+    c = torch.tensor(centers[ic[pick]] @ np.asarray(case["cell"]), device="cuda")
+    u = torch.tensor(axes[ik[pick]], device="cuda")
+    out = {}
+    for prec in ("fp64", "mixed"):
+        cc = c.clone().requires_grad_(True)
+        uu = u.clone().requires_grad_(True)
+        e = pose_energy(plan, cc, uu, prec)
+        e.sum().backward()
+        out[prec] = (e.detach().cpu().numpy(), cc.grad.cpu().numpy(), uu.grad.cpu().numpy())
+    m = oracle.error_metrics(out["fp64"][0], out["mixed"][0])
+    print(f"pore poses: {m}", flush=True)
+    assert m["accessible_poses"] > 50_000
+    assert m["accessible_max_abs_error_K"] < 1e-3
+    for g64, gmx in zip(out["fp64"][1:], out["mixed"][1:]):
+        scale = np.maximum(np.abs(g64).max(axis=1, keepdims=True), 1.0)
+        assert (np.abs(gmx - g64) / scale).max() < 1e-4
+    # the FP64 pose energies are the field's
+    np.testing.assert_allclose(out["fp64"][0], e_field[ic[pick], ik[pick]], rtol=1e-10, atol=1e-8)
