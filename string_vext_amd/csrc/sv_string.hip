@@ -86,7 +86,7 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 __device__ __forceinline__ double grid_floor(double v) { return floor(v / kRound) * kRound; }
 
 // Candidate lists of one framework for string images: a fixed grid of bins
-// over the cell (bins of about 2 A), each with the tile-relative list of every
+// over the cell (bins of about 1 A, kMinBinEdge), each with the tile-relative list of every
 // image within (bin radius + rc + max|o|) of its centre, built once.
 struct ImageGrid {
   const Job* job;            // coefficients, cell, list layout (segment = bin * n_types + type)
@@ -115,8 +115,8 @@ __device__ __forceinline__ bool decision_in_doubt(double v0, double v2) {
   return !(fabs(v2 - v0) > tol);   // nonfinite values are in doubt too
 }
 
-// Type count up to which the FP64 image sweep runs as one stream over the
-// bin's type segments (more types: one sweep per type).
+// Type count up to which an image sweep runs as one stream over the bin's
+// type segments (more types: one sweep per type).
 constexpr int kStreamTypes = 16;
 
 // FP64 re-evaluation of the images whose decision is in doubt: waves
@@ -139,10 +139,14 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
                                                     double* __restrict__ V, double* __restrict__ G,
                                                     Recheck rc = Recheck{}) {
   __shared__ int s_near[4][128];   // queued near-contact candidates per wave
-  // FP64 sweep: per-type coefficients (c12, c6, shift, floor2) and the bin's
-  // type segment bounds, per wave
+  // streamed sweeps: per-type FP64 coefficients (c12, c6, shift, floor2) and
+  // the bin's type segment bounds, per wave
   __shared__ double4 s_coef[4][kStreamTypes];
   __shared__ long long s_bound[4][kStreamTypes + 1];
+  // mixed sweep: per-type FP32 constants (sigma^6 s^3, 6 c6 s^4, near-contact
+  // radius^2) and the FP64 scale c6 s^3 of the type's FP32 partials
+  __shared__ float4 s_cf[4][kStreamTypes];
+  __shared__ double s_c6x[4][kStreamTypes];
   const int lane = threadIdx.x & 63;
   const int wv = uniform_i(threadIdx.x >> 6);
   const int slot = blockIdx.x * 4 + wv;   // output index
@@ -212,8 +216,103 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
     }
   };
   const int n_types = J.n_types;
-  const bool stream = !MIXED && n_types <= kStreamTypes;
-  if (stream) {
+  const bool stream = n_types <= kStreamTypes;
+  if (stream && MIXED) {
+    // The mixed sweep as one stream over the bin's type segments.  A lane's
+    // FP32 partials cover a run of candidates of one type (its candidates
+    // come in list order, so its type only grows); they go into the FP64 sums
+    // with the type's FP64 scale when the type changes and at the end, as the
+    // per-type sweep flushes them.  Queued near-contact candidates find their
+    // type from the segment bounds when they are evaluated.
+    if (lane < n_types) {
+      s_coef[wv][lane] = make_double4(J.c12[lane], J.c6[lane], J.shift[lane], J.floor2[lane]);
+      const float s6x = J.s6xf[lane];
+      s_cf[wv][lane] = make_float4(s6x, (float)(6.0 * J.c6x[lane] * (double)J.inv_rc2f),
+                                   fmaxf(J.near2f[lane], J.lo2f[lane]), 0.0f);
+      s_c6x[wv][lane] = J.c6x[lane];
+    }
+    if (lane <= n_types) s_bound[wv][lane] = IG.offsets[seg0 + lane];
+    wave_barrier_s();
+    const long long beg = s_bound[wv][0], end = s_bound[wv][n_types];
+    const float sc = J.inv_rc2f;
+    const float ox[2] = {J.o0sxf, J.o1sxf}, mx[2] = {J.m2o0xf, J.m2o1xf};
+    const float of0 = (float)o[0], of1 = (float)o[1];
+    int nq = 0;
+    int ty = 0, run_ty = 0, slots = 0;   // the type of the lane's current partials
+    float eacc = 0.f, gcf[3] = {0.f, 0.f, 0.f}, guf[3] = {0.f, 0.f, 0.f};
+    auto flush = [&]() {
+      E += fma(s_c6x[wv][run_ty], (double)eacc, 2.0 * s_coef[wv][run_ty].z * slots);
+      eacc = 0.f;
+      slots = 0;
+      if (GRAD)
+        for (int k = 0; k < 3; ++k) {
+          gc[k] += (double)gcf[k];
+          gu[k] += (double)guf[k];
+          gcf[k] = guf[k] = 0.f;
+        }
+    };
+    float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (beg + lane < end) next = IG.list32[beg + lane];
+    for (long long b0 = beg; b0 < end; b0 += 64) {
+      const long long i = b0 + lane;
+      const float4 e = next;
+      if (i + 64 < end) next = IG.list32[i + 64];
+      float rf[3] = {0.f, 0.f, 0.f}, rrf = 3.0e38f;
+      if (i < end) {
+        rf[0] = e.x - cf[0];
+        rf[1] = e.y - cf[1];
+        rf[2] = e.z - cf[2];
+        rrf = rf[0] * rf[0] + rf[1] * rf[1] + rf[2] * rf[2];
+        while (i >= s_bound[wv][ty + 1]) ++ty;   // i < end = s_bound[n_types]: ty < n_types
+      }
+      if (ty != run_ty) {
+        flush();
+        run_ty = ty;
+      }
+      const float4 k4 = s_cf[wv][ty];   // (s6x, c6f6, near2x, -)
+      const bool near = rrf < k4.z;     // false past the list's end (rrf = 3e38)
+      const unsigned long long mnear = __ballot(near);
+      if (near) s_near[wv][nq + lanes_below(mnear)] = (int)(i - beg);
+      nq += __popcll(mnear);
+      ++slots;
+      const float rr = near ? 3.0e38f : rrf;
+      const float tt = rf[0] * uf[0] + rf[1] * uf[1] + rf[2] * uf[2];
+#pragma unroll
+      for (int si = 0; si < 2; ++si) {
+        const float x = __builtin_amdgcn_fmed3f(fmaf(tt, mx[si], fmaf(rr, sc, ox[si])), 0.0f, 1.0f);
+        const float iv = __builtin_amdgcn_rcpf(x);
+        const float i3 = iv * iv * iv;
+        eacc = fmaf(i3, fmaf(i3, k4.x, -1.0f), eacc);
+        if (GRAD) {
+          // no gradient beyond the cutoff (x clamped to 1)
+          const float g = x < 1.0f ? k4.y * (i3 * iv * fmaf(-2.0f * k4.x, i3, 1.0f)) : 0.0f;
+          const float of = si ? of1 : of0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const float dk = fmaf(of, uf[k], -rf[k]);
+            gcf[k] = fmaf(g, dk, gcf[k]);
+            guf[k] = fmaf(g * of, dk, guf[k]);
+          }
+        }
+      }
+      if (nq >= 64 || b0 + 64 >= end) {
+        wave_barrier_s();
+        for (int q0 = 0; q0 < nq; q0 += 64)
+          if (q0 + lane < nq) {
+            const long long id = beg + s_near[wv][q0 + lane];
+            int qt = 0;
+            while (id >= s_bound[wv][qt + 1]) ++qt;
+            const double4 co = s_coef[wv][qt];
+            const double qv[3] = {IG.list64[3 * id], IG.list64[3 * id + 1], IG.list64[3 * id + 2]};
+            pair64r(qv, co.x, co.y, co.z, co.w);
+          }
+        wave_barrier_s();
+        nq = 0;
+      }
+    }
+    flush();
+  }
+  if (stream && !MIXED) {
     // One sweep over the bin's type segments, which are contiguous: each lane
     // finds its candidate's type from the segment bounds and takes the type's
     // coefficients from LDS, so the sweep has no break per type and the next
