@@ -599,7 +599,7 @@ __global__ __launch_bounds__(256) void string_finish_kernel(StringParams sp,
   const int N = sp.n_images;
   const int P = sp.n_strings * N;
   double* ang = sh;                 // [N][3] checked s2 angles
-  double* red = sh + 3 * N;         // [2 * blockDim]
+  double* red = sh + 3 * N;         // [2 * waves]
   const int base = s * N;
   double dc = 0.0, dor = 0.0;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
@@ -617,17 +617,20 @@ __global__ __launch_bounds__(256) void string_finish_kernel(StringParams sp,
     dc += sqrt(sq);
     dor += sqrt(so);
   }
-  red[threadIdx.x] = dc;
-  red[blockDim.x + threadIdx.x] = dor;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      red[threadIdx.x] += red[threadIdx.x + w];
-      red[blockDim.x + threadIdx.x] += red[blockDim.x + threadIdx.x + w];
-    }
-    __syncthreads();
+  // wave sums, then the waves' sums in wave order
+  dc = wave_sum_d(dc);
+  dor = wave_sum_d(dor);
+  const int wv = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wv] = dc;
+    red[n_waves + wv] = dor;
   }
-  const double tot_c = red[0], tot_o = red[blockDim.x];
+  __syncthreads();
+  double tot_c = 0.0, tot_o = 0.0;
+  for (int w = 0; w < n_waves; ++w) {
+    tot_c += red[w];
+    tot_o += red[n_waves + w];
+  }
   const int it = iterations[s];
   const bool check = it % kCheckEvery == 0 && it >= kCheckEvery;
   const bool done = check && isfinite(tot_c) && isfinite(tot_o) && tot_c < sp.conv_c &&
