@@ -622,13 +622,20 @@ def field_line(args):
     f64_steps = max(1, args.fp64_steps)
     step64, field64, run64, t64 = time_field(plan, poses, "fp64", f64_steps, 1, case["temperature"])
     log(f"fp64: {step64 * 1e3:.3f} ms/step, field kernel {field64:.3f} ms")
+    # the FP64 form with 1/d^2 from v_rcp_f64 + one Newton step (a stricter
+    # denominator than the reference's IEEE division)
+    step64r, field64r, _, _ = time_field(plan, poses, "fp64_rcp", f64_steps, 1, case["temperature"])
+    log(f"fp64_rcp: {step64r * 1e3:.3f} ms/step, field kernel {field64r:.3f} ms")
 
-    errors = porous = None
+    errors = porous = rcp_rel = None
     if not args.no_check:
         e64 = energies(plan, poses, "fp64")
         emx = energies(plan, poses, "mixed")
         errors = error_metrics(e64, emx)
         log(f"mixed vs fp64: {errors}")
+        rcp_rel = float((np.abs(energies(plan, poses, "fp64_rcp") - e64)
+                         / np.maximum(np.abs(e64), 1.0)).max())
+        log(f"fp64_rcp vs fp64: max relative difference {rcp_rel:.3e} (1 K floor)")
         porous = porous_field_audit(tile_centers)
         log(f"porous field, mixed vs fp64: {porous}")
 
@@ -683,6 +690,10 @@ def field_line(args):
         "fp64": {"value": pairs / step64, "ms_per_step": step64 * 1e3, "kernel_ms": field64,
                  "frac_of_fp64_peak": pairs * FLOP_PER_PAIR / (field64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
         "speedup_mixed_vs_fp64": step64 / step_s,
+        "fp64_rcp": {"value": pairs / step64r, "ms_per_step": step64r * 1e3, "kernel_ms": field64r,
+                     "frac_of_fp64_peak": pairs * FLOP_PER_PAIR / (field64r * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                     "max_relative_vs_fp64_floor1K": rcp_rel,
+                     "speedup_mixed_vs_fp64_rcp": step64r / step_s},
         "error_vs_fp64": errors,
         "porous_field_error_vs_fp64": porous,
         "phases_ms": t_mixed,

@@ -121,7 +121,10 @@ int sv_poses_grid_ex(void *plan, int size, int n_axes, const double *axes, int t
 /* ------------------------------------------------------------------------ */
 /* Whole-field evaluation, host outputs (synchronous).                       */
 /*   precision   0 = FP64 (reference arithmetic), 1 = mixed FP32 pair math   */
-/*               with FP64 accumulation and FP64 near-contact terms          */
+/*               with FP64 accumulation and FP64 near-contact terms,         */
+/*               2 = FP64 with 1/d^2 from the hardware reciprocal and one    */
+/*               Newton step instead of IEEE division (fields and batches;   */
+/*               the pose and string entry points take 0 or 1)               */
 /*   energy      [n_centers][n_axes] FP64   (reference kernel layout) or NULL */
 /*   orientation [n_axes][n_centers] FP32   (vext.py:31 orientation_K) or NULL*/
 /*   marginal    [n_centers] FP32  -T(logsumexp(-E/T) - ln K) (physics.py:126)*/

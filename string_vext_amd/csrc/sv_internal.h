@@ -222,7 +222,9 @@ hipError_t launch_legacy(int n_centers, int n_axes, double cutoff2, const double
                          const double* floor2, const long long* starts, const int* indices,
                          double* output, int* bad_center, hipStream_t s);
 
-// Precision selectors for launch_field.
-enum { PREC_FP64 = 0, PREC_MIXED = 1 };
+// Precision selectors for launch_field.  PREC_FP64_RCP (fields only) is the
+// FP64 form with 1/d^2 from v_rcp_f64 and one Newton step instead of IEEE
+// division.
+enum { PREC_FP64 = 0, PREC_MIXED = 1, PREC_FP64_RCP = 2 };
 
 }  // namespace sv

@@ -433,7 +433,9 @@ __device__ __forceinline__ void write_pose(const WaveCtx& c, double E, bool with
 }
 
 // FP64 pair term with the reference's exact cutoff/floor semantics
-// (kernel.cpp:29-36).  d = o u - r where r = image - center.
+// (kernel.cpp:29-36).  d = o u - r where r = image - center.  RCP takes 1/d^2
+// from v_rcp_f64 and one Newton step instead of the IEEE division.
+template <bool RCP = false>
 __device__ __forceinline__ double pair64(double o, double ux, double uy, double uz, double rx,
                                          double ry, double rz, double rc2, double c12, double c6,
                                          double sh, double fl2) {
@@ -442,7 +444,7 @@ __device__ __forceinline__ double pair64(double o, double ux, double uy, double 
   double e = 0.0;
   if (d2 < rc2) {
     const double safe = fmax(d2, fl2);   // the floor: one v_max_f64
-    const double inv = 1.0 / safe;
+    const double inv = RCP ? inv_fast(safe) : 1.0 / safe;
     const double i6 = inv * inv * inv;
     e = c12 * i6 * i6 - c6 * i6 + sh;
   }
@@ -829,6 +831,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void f
   }
 }
 
+template <bool RCP>
 __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__ jobs, int n_jobs,
                                                          const long long* __restrict__ job_work,
                                                          long long n_work,
@@ -862,14 +865,14 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
       if (keep) {
         const int p = lanes_below(m);
         s_r[wv][0][p] = rx; s_r[wv][1][p] = ry; s_r[wv][2][p] = rz;
-        if (do_site) S += pair64(0.0, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2);
+        if (do_site) S += pair64<RCP>(0.0, ux, uy, uz, rx, ry, rz, rc2, c12, c6, sh, fl2);
       }
       wave_sync();
       const int n = __popcll(m);
       for (int q = 0; q < n; ++q) {
         const double ax = s_r[wv][0][q], ay = s_r[wv][1][q], az = s_r[wv][2][q];
-        E += pair64(o0, ux, uy, uz, ax, ay, az, rc2, c12, c6, sh, fl2);
-        E += pair64(o1, ux, uy, uz, ax, ay, az, rc2, c12, c6, sh, fl2);
+        E += pair64<RCP>(o0, ux, uy, uz, ax, ay, az, rc2, c12, c6, sh, fl2);
+        E += pair64<RCP>(o1, ux, uy, uz, ax, ay, az, rc2, c12, c6, sh, fl2);
       }
       wave_sync();
     }
@@ -1086,9 +1089,12 @@ hipError_t launch_field(int precision, const Job* d_jobs, int n_jobs, const long
     else
       hipLaunchKernelGGL(field_mixed_kernel<false>, grid, block, 0, s, d_jobs, n_jobs, d_job_work,
                          n_work, d_offsets, d_list32, d_list64);
+  } else if (precision == PREC_FP64_RCP) {
+    hipLaunchKernelGGL(field_fp64_kernel<true>, grid, block, 0, s, d_jobs, n_jobs, d_job_work,
+                       n_work, d_offsets, d_list64);
   } else {
-    hipLaunchKernelGGL(field_fp64_kernel, grid, block, 0, s, d_jobs, n_jobs, d_job_work, n_work,
-                       d_offsets, d_list64);
+    hipLaunchKernelGGL(field_fp64_kernel<false>, grid, block, 0, s, d_jobs, n_jobs, d_job_work,
+                       n_work, d_offsets, d_list64);
   }
   return hipGetLastError();
 }

@@ -304,7 +304,8 @@ int batch_build(int n_jobs, Plan* const* plans, Poses* const* poses, int precisi
                 const double* temps, double* const* energy, float* const* orientation,
                 float* const* marginal, float* const* grid, float* const* site, Batch** out) {
   if (n_jobs < 1) return fail(1, "at least one job required");
-  if (precision != PREC_FP64 && precision != PREC_MIXED) return fail(1, "precision must be 0 or 1");
+  if (precision != PREC_FP64 && precision != PREC_MIXED && precision != PREC_FP64_RCP)
+    return fail(1, "precision must be 0, 1 or 2");
   const int device = plans[0]->device;
   auto B = new Batch();
   B->device = device;

@@ -16,7 +16,10 @@ import numpy as np
 from . import _lib
 from .physics import body_offsets
 
-PRECISIONS = {"fp64": 0, "mixed": 1}
+# "fp64_rcp" (fields and batches only): FP64 with 1/d^2 from the hardware
+# reciprocal and one Newton step instead of IEEE division.
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp64_rcp": 2}
+POSE_PRECISIONS = ("fp64", "mixed")
 DEFAULT_NEAR_ENERGY_K = 50.0
 
 _P = ctypes.c_void_p
@@ -33,9 +36,9 @@ def _f64(a, shape=None):
     return a
 
 
-def precision_code(precision) -> int:
-    if precision not in PRECISIONS:
-        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+def precision_code(precision, allowed=tuple(PRECISIONS)) -> int:
+    if precision not in allowed:
+        raise ValueError(f"precision must be one of {sorted(allowed)}")
     return PRECISIONS[precision]
 
 
@@ -305,7 +308,7 @@ class PoseGrid:
         """All arguments after ``n`` are device pointers (ints) or None."""
         self.binned_token = None
         _lib.check(self.lib.sv_pose_grid_eval(self.handle, int(n), centers, axes,
-                                              precision_code(precision), grad_out, energy,
+                                              precision_code(precision, POSE_PRECISIONS), grad_out, energy,
                                               grad_center, grad_axis))
         self.binned_token = token
 
@@ -317,7 +320,7 @@ class PoseGrid:
         ``token`` (any object) is kept as ``binned_token`` to name these poses."""
         self.binned_token = None
         _lib.check(self.lib.sv_pose_grid_eval_async(self.handle, stream or None, int(n), centers,
-                                                    axes, precision_code(precision), grad_out,
+                                                    axes, precision_code(precision, POSE_PRECISIONS), grad_out,
                                                     energy, grad_center, grad_axis))
         self.binned_token = token
 
@@ -326,7 +329,7 @@ class PoseGrid:
         """Evaluate the poses of the last eval again (new outputs) from the
         grid's binned copies, without binning them again."""
         _lib.check(self.lib.sv_pose_grid_reeval_async(self.handle, stream or None,
-                                                      precision_code(precision), grad_out, energy,
+                                                      precision_code(precision, POSE_PRECISIONS), grad_out, energy,
                                                       grad_center, grad_axis))
 
     def check(self):
@@ -392,7 +395,7 @@ class PoseList:
 
     def eval(self, precision="fp64", grad_out=None, energy=None, grad_center=None, grad_axis=None):
         """Device pointers (ints) for every array; returns nothing."""
-        _lib.check(self.lib.sv_pose_list_eval(self.handle, precision_code(precision), grad_out,
+        _lib.check(self.lib.sv_pose_list_eval(self.handle, precision_code(precision, POSE_PRECISIONS), grad_out,
                                               energy, grad_center, grad_axis))
 
     def energy_host(self, precision="fp64", grad=False):

@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .field import Plan, PoseList, _ptr, precision_code
+from .field import POSE_PRECISIONS, Plan, PoseList, _ptr, precision_code
 
 MOVE_FRAC = 1e-4                    # delta_frac of the reference inputs (input.dat)
 MOVE_ANGLE = np.radians(1.0)        # delta_angle_degree 1.0
@@ -48,7 +48,8 @@ class StringSystem:
         h = ctypes.c_void_p()
         _lib.check(self.lib.sv_strings_create(self.n_strings, handles, self.n_images, _ptr(init),
                                               _ptr(dirs), float(move_frac), float(move_angle),
-                                              precision_code(precision), ctypes.byref(h)))
+                                              precision_code(precision, POSE_PRECISIONS),
+                                              ctypes.byref(h)))
         self.handle = h.value
 
     def run(self, iterations, graph=True):
