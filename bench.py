@@ -347,7 +347,10 @@ def poses_line(args):
         u /= np.linalg.norm(u, axis=1, keepdims=True)
         c = torch.tensor(frac @ case["cell"], device="cuda")
         data.append((plan, c, torch.tensor(u, device="cuda")))
-    for prec in ("mixed", "fp64"):
+    # "mixed_fused": energy and gradient from one site pass in the forward
+    # (pose_energy(..., fused=True)); the backward only scales them
+    modes = {"mixed": ("mixed", False), "fp64": ("fp64", False), "mixed_fused": ("mixed", True)}
+    for name, (prec, fused) in modes.items():
         times, outs = [], []
         for rep in range(args.warmup + args.steps):
             torch.cuda.synchronize()
@@ -356,32 +359,32 @@ def poses_line(args):
             for plan, c, u in data:
                 cc = c.clone().requires_grad_(True)
                 uu = u.clone().requires_grad_(True)
-                e = pose_energy(plan, cc, uu, prec)
+                e = pose_energy(plan, cc, uu, prec, fused=fused)
                 e.sum().backward()
                 step_out.append((e.detach(), cc.grad, uu.grad))
             torch.cuda.synchronize()
             if rep >= args.warmup:
                 times.append(time.perf_counter() - t)
             outs = step_out
-        results[prec] = (float(np.mean(times)), outs)
+        results[name] = (float(np.mean(times)), outs)
     # site-kernel durations (HIP events on torch's stream) and the pair census
-    kernels = {prec: {"binning": 0.0, "forward": 0.0, "backward": 0.0, "combine": 0.0}
-               for prec in ("mixed", "fp64")}
+    kernels = {name: {"binning": 0.0, "forward": 0.0, "backward": 0.0, "combine": 0.0}
+               for name in modes}
     pairs = 0
-    for prec in ("mixed", "fp64"):
+    for name, (prec, fused) in modes.items():
         for plan, c, u in data:
             grid = _grid_for(plan, len(c))
             grid.set_timing(True)
             cc = c.clone().requires_grad_(True)
             uu = u.clone().requires_grad_(True)
-            e = pose_energy(plan, cc, uu, prec)
+            e = pose_energy(plan, cc, uu, prec, fused=fused)
             fwd = grid.timings()
-            if prec == "mixed":
+            if name == "mixed":
                 pairs += grid.census()
             e.sum().backward()
-            bwd = grid.timings()
+            bwd = grid.timings() if not fused else {"site_kernel": 0.0, "combine": 0.0}
             grid.set_timing(False)
-            kt = kernels[prec]
+            kt = kernels[name]
             kt["binning"] += fwd["binning"]
             kt["forward"] += fwd["site_kernel"]
             kt["backward"] += bwd["site_kernel"]
@@ -409,6 +412,11 @@ def poses_line(args):
     mk = kernels["mixed"]
     fwd_tf = pairs * FLOP_PER_PAIR / (mk["forward"] * 1e-3) / 1e12
     bwd_tf = pairs * FLOP_PER_GRAD_PAIR / (mk["backward"] * 1e-3) / 1e12
+    fused_tf = pairs * FLOP_PER_GRAD_PAIR / (kernels["mixed_fused"]["forward"] * 1e-3) / 1e12
+    fused_diff = 0.0
+    for (e1, gc1, gu1), (e2, gc2, gu2) in zip(results["mixed"][1], results["mixed_fused"][1]):
+        fused_diff = max(fused_diff, float((e1 - e2).abs().max()), float((gc1 - gc2).abs().max()),
+                         float((gu1 - gu2).abs().max()))
     return {"metric": "poses/s, energy + dE/d(center, director) via torch.autograd, 8 x 65,536 poses",
             "value": n / results["mixed"][0], "unit": "poses/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": results["mixed"][0] * 1e3,
@@ -426,6 +434,12 @@ def poses_line(args):
                                       "kernel_ms_per_framework": mk["backward"] / len(data)}},
             "phases_ms_per_framework": {prec: {k: v / len(data) for k, v in kt.items()}
                                         for prec, kt in kernels.items()},
+            "fused": {"value": n / results["mixed_fused"][0],
+                      "ms_per_step": results["mixed_fused"][0] * 1e3,
+                      "kernel": "site_kernel<mixed, energy + gradient>", "achieved": fused_tf,
+                      "frac": fused_tf / PEAK_FP32_TFLOPS, "flop_per_pair": FLOP_PER_GRAD_PAIR,
+                      "kernel_ms_per_framework": kernels["mixed_fused"]["forward"] / len(data),
+                      "max_abs_diff_vs_two_pass": fused_diff},
             "fp64": {"value": n / results["fp64"][0], "ms_per_step": results["fp64"][0] * 1e3},
             "speedup_mixed_vs_fp64": results["fp64"][0] / results["mixed"][0],
             "energy_error_vs_fp64": {
@@ -746,7 +760,7 @@ def main():
         log(f"ragged: {line['ragged']['value']:.3e} pair-evals/s")
         sub.steps, sub.warmup = 5, 2
         line["poses"] = compact(poses_line(sub), (
-            "config", "roofline", "phases_ms_per_framework", "fp64", "energy_error_vs_fp64",
+            "config", "roofline", "phases_ms_per_framework", "fused", "fp64", "energy_error_vs_fp64",
             "gradient_error_vs_fp64_rel", "central_difference_rel_error_h1e-4"))
         log(f"poses: {line['poses']['value']:.3e} poses/s")
         sl = strings_line(args)

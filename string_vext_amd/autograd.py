@@ -9,13 +9,20 @@ free parameters, and autograd carries the normalization).  The poses stay on
 the device: each call bins them into the plan's cached tile grid on the GPU,
 enqueued on torch's current stream without a host synchronization
 (sv_pose_grid_eval_async); invalid poses give NaN energies, like other torch
-operations, and ``check_poses(plan)`` raises on them.  Forward runs the energy
-kernel; backward runs a separate kernel that recomputes every pair term and
-returns dE/dc and dE/du scaled by the upstream gradient.  When no other
-evaluation has used the grid since this forward, backward reuses the
-forward's binning (sv_pose_grid_reeval_async).  There is no counterpart in the reference, whose String
-forces are forward differences of the energy
-(native/string/source_code/global.h:1281-1324).
+operations, and ``check_poses(plan)`` raises on them.
+
+Two ways to the gradient:
+  * default: forward runs the energy kernel; backward runs a separate kernel
+    that recomputes every pair term and returns dE/dc and dE/du scaled by the
+    upstream gradient.  When no other evaluation has used the grid since this
+    forward, backward reuses the forward's binning (sv_pose_grid_reeval_async);
+  * ``fused=True``: when an input requires a gradient, forward runs one kernel
+    that sums each site's energy and gradient together and keeps the pose
+    gradients dE/dc, dE/du (6 doubles per pose); backward only scales them by
+    the upstream gradient.  One pass over the pair terms instead of two.
+
+There is no counterpart in the reference, whose String forces are forward
+differences of the energy (native/string/source_code/global.h:1281-1324).
 """
 from __future__ import annotations
 
@@ -35,7 +42,7 @@ def _grid_for(plan: Plan, n: int) -> PoseGrid:
 
 class PoseEnergy(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, centers, axes, plan: Plan, precision: str):
+    def forward(ctx, centers, axes, plan: Plan, precision: str, fused: bool = False):
         if centers.shape != axes.shape or centers.ndim != 2 or centers.shape[1] != 3:
             raise ValueError("centers and axes must both be (n, 3)")
         if len(centers) == 0:
@@ -47,9 +54,18 @@ class PoseEnergy(torch.autograd.Function):
         grid = _grid_for(plan, len(c))
         stream = torch.cuda.current_stream(device).cuda_stream
         token = object()
-        grid.eval_async(stream, len(c), c.data_ptr(), u.data_ptr(), precision,
-                        energy=energy.data_ptr(), token=token)
-        ctx.save_for_backward(c, u)
+        ctx.fused = bool(fused) and any(ctx.needs_input_grad[:2])
+        if ctx.fused:
+            grad_c = torch.empty((len(c), 3), dtype=torch.float64, device=device)
+            grad_u = torch.empty((len(c), 3), dtype=torch.float64, device=device)
+            grid.eval_async(stream, len(c), c.data_ptr(), u.data_ptr(), precision,
+                            energy=energy.data_ptr(), grad_center=grad_c.data_ptr(),
+                            grad_axis=grad_u.data_ptr(), token=token)
+            ctx.save_for_backward(grad_c, grad_u)
+        else:
+            grid.eval_async(stream, len(c), c.data_ptr(), u.data_ptr(), precision,
+                            energy=energy.data_ptr(), token=token)
+            ctx.save_for_backward(c, u)
         ctx.grid = grid
         ctx.token = token
         ctx.precision = precision
@@ -58,6 +74,10 @@ class PoseEnergy(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_energy):
+        if ctx.fused:
+            grad_c, grad_u = ctx.saved_tensors
+            g = grad_energy.detach().to(device=grad_c.device, dtype=torch.float64)[:, None]
+            return (g * grad_c).to(ctx.dtypes[0]), (g * grad_u).to(ctx.dtypes[1]), None, None, None
         c, u = ctx.saved_tensors
         n = len(c)
         g = grad_energy.detach().to(device=c.device, dtype=torch.float64).contiguous()
@@ -73,7 +93,7 @@ class PoseEnergy(torch.autograd.Function):
             ctx.grid.eval_async(stream, n, c.data_ptr(), u.data_ptr(), ctx.precision,
                                 grad_out=g.data_ptr(), grad_center=grad_c.data_ptr(),
                                 grad_axis=grad_u.data_ptr())
-        return grad_c.to(ctx.dtypes[0]), grad_u.to(ctx.dtypes[1]), None, None
+        return grad_c.to(ctx.dtypes[0]), grad_u.to(ctx.dtypes[1]), None, None, None
 
 
 def check_poses(plan: Plan) -> None:
@@ -84,6 +104,6 @@ def check_poses(plan: Plan) -> None:
 
 
 def pose_energy(plan: Plan, centers: torch.Tensor, axes: torch.Tensor,
-                precision: str = "fp64") -> torch.Tensor:
+                precision: str = "fp64", fused: bool = False) -> torch.Tensor:
     """Differentiable energies of independent poses (center_i, director_i)."""
-    return PoseEnergy.apply(centers, axes, plan, precision)
+    return PoseEnergy.apply(centers, axes, plan, precision, fused)

@@ -16,13 +16,14 @@
 //     holds every image within (bin circumradius + rc) of its centre, split per
 //     LJ type into class 0 -- images a member site can bring within the
 //     near-contact radius -- and class 1, the rest;
-//   * within a bin the sites come in octant order, and a bin of n sites is cut
-//     into ceil(n / 64) chunks of nearly equal size, so a chunk covers part of
-//     its bin;
-//   * a block of kSiteWaves waves takes one chunk (one lane per site).  The
+//   * within a bin the sites are ordered by the Morton index of their cell of
+//     the bin's 4 x 4 x 4 sub-grid, and the cells are grouped greedily into
+//     runs of whole cells holding at most 64 sites (a cell of more than 64 is
+//     cut into equal parts), so a run covers a compact part of its bin;
+//   * a block of kSiteWaves waves takes one run (one lane per site).  The
 //     waves deal the list's 64-entry batches round robin over all (type,
 //     class) segments; the lane that loads an entry keeps it only if it lies
-//     within rc of the chunk's bounding sphere, and the kept entries are
+//     within rc of the run's bounding box, and the kept entries are
 //     compacted into LDS (ballot + mbcnt), where every lane reads them by
 //     broadcast in unrolled groups; the waves' sums are added in wave order.
 //
@@ -219,19 +220,21 @@ __global__ __launch_bounds__(64 * kSiteWaves) void site_kernel(const Job* __rest
                pz = sa.site_rel[3 * slot + 2];
   const double scale = J.list_scale;
   float pxf = (float)(px * scale), pyf = (float)(py * scale), pzf = (float)(pz * scale);
-  // Bounding sphere of the run's sites (units of rc; lanes past the count
-  // repeat the first site, NaN sites of invalid poses drop out of fmin/fmax):
-  // an entry farther than its radius + 1 from its centre is within rc of no
-  // site.  A run of whole cells has a fixed set of sites whatever slots the
-  // binning gave them, so its sphere and every lane's result are fixed too; a
-  // part of an overfull cell keeps the bin's whole list.
-  const float cxf = uniform_f(0.5f * (wave_min_f(pxf) + wave_max_f(pxf)));
-  const float cyf = uniform_f(0.5f * (wave_min_f(pyf) + wave_max_f(pyf)));
-  const float czf = uniform_f(0.5f * (wave_min_f(pzf) + wave_max_f(pzf)));
-  const float dcx = pxf - cxf, dcy = pyf - cyf, dcz = pzf - czf;
-  const float reach = uniform_f(sqrtf(wave_max_f(fmaf(dcz, dcz, fmaf(dcy, dcy, dcx * dcx))))) +
-                      1.0f + 1e-4f;
-  const float reach2 = run.whole ? reach * reach : INFINITY;
+  // Bounding box of the run's sites (units of rc; lanes past the count repeat
+  // the first site, NaN sites of invalid poses drop out of fmin/fmax): an
+  // entry whose distance to the box exceeds 1 is within rc of no site (per
+  // axis, the distance to the box is at most the distance to any site in
+  // it).  The margin 1e-4 covers the FP32 rounding of the box.  A run of whole
+  // cells has a fixed set of sites whatever slots the binning gave them, so
+  // its box and every lane's result are fixed too; a part of an overfull cell
+  // keeps the bin's whole list.
+  const float lox = wave_min_f(pxf), hix = wave_max_f(pxf);
+  const float loy = wave_min_f(pyf), hiy = wave_max_f(pyf);
+  const float loz = wave_min_f(pzf), hiz = wave_max_f(pzf);
+  const float cxf = uniform_f(0.5f * (lox + hix)), hx = uniform_f(0.5f * (hix - lox));
+  const float cyf = uniform_f(0.5f * (loy + hiy)), hy = uniform_f(0.5f * (hiy - loy));
+  const float czf = uniform_f(0.5f * (loz + hiz)), hz = uniform_f(0.5f * (hiz - loz));
+  const float reach2 = run.whole ? (1.0f + 1e-4f) * (1.0f + 1e-4f) : INFINITY;
   // keep the FP32 site in registers (left alone, the compiler converts it again per batch)
   asm volatile("" : "+v"(pxf), "+v"(pyf), "+v"(pzf));
   const double rc2 = J.rc2;
@@ -263,7 +266,9 @@ __global__ __launch_bounds__(64 * kSiteWaves) void site_kernel(const Job* __rest
         float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
         if (idx < end) {
           e = list32[idx];
-          const float ex = e.x - cxf, ey = e.y - cyf, ez = e.z - czf;
+          const float ex = fmaxf(fabsf(e.x - cxf) - hx, 0.0f);
+          const float ey = fmaxf(fabsf(e.y - cyf) - hy, 0.0f);
+          const float ez = fmaxf(fabsf(e.z - czf) - hz, 0.0f);
           keep = fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= reach2;
         }
         const unsigned long long kept = __ballot(keep);

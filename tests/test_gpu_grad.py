@@ -170,6 +170,43 @@ def test_autograd_function(oracle, case, poses):
     assert radial < 1e-6 * max(1.0, v.grad.abs().max().item())
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autograd_fused_gradient_equals_two_pass(oracle, case, poses, precision):
+    """pose_energy(..., fused=True): energy and gradient from one site pass in the
+    forward, scaled by the upstream gradient in the backward.  Same site sums as
+    the energy kernel and the recomputing backward kernel, so the results are
+    those of the two-pass route; FP64 also against the oracle's gradient."""
+    import torch
+    from string_vext_amd.autograd import check_poses, pose_energy
+    centers, axes = poses
+    plan = _plan(case)
+    w = torch.linspace(0.5, 1.5, len(centers), dtype=torch.float64, device="cuda")
+    out = {}
+    for fused in (False, True):
+        c = torch.tensor(centers, dtype=torch.float64, device="cuda", requires_grad=True)
+        u = torch.tensor(axes, dtype=torch.float64, device="cuda", requires_grad=True)
+        e = pose_energy(plan, c, u, precision, fused=fused)
+        (e * w).sum().backward()
+        check_poses(plan)
+        out[fused] = [t.detach().cpu().numpy() for t in (e, c.grad, u.grad)]
+    for a, b in zip(out[False], out[True]):
+        print(f"fused vs two-pass max abs diff {np.abs(b - a).max():.3e}", flush=True)
+        np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-12 * np.abs(a).max())
+    if precision == "fp64":
+        e_ref, gc_ref, gu_ref = oracle.pose_grad(case, centers, axes)
+        wn = w.cpu().numpy()[:, None]
+        np.testing.assert_allclose(out[True][0], e_ref, rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(out[True][1], wn * gc_ref, rtol=1e-9,
+                                   atol=1e-9 * np.abs(gc_ref).max())
+        np.testing.assert_allclose(out[True][2], wn * gu_ref, rtol=1e-9,
+                                   atol=1e-9 * np.abs(gu_ref).max())
+    # no input needs a gradient: the fused flag runs the energy kernel alone
+    with torch.no_grad():
+        e0 = pose_energy(plan, torch.tensor(centers, device="cuda"),
+                         torch.tensor(axes, device="cuda"), precision, fused=True)
+    np.testing.assert_array_equal(e0.cpu().numpy(), out[False][0])
+
+
 def test_pose_grid_device_binning_matches_oracle_and_host_binning(oracle, case, poses):
     """sv_pose_grid_eval: poses binned on the GPU from device arrays, against the
     FP64 oracle (energy and gradient) and against the host-binned pose list."""
