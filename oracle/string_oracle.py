@@ -9,6 +9,14 @@ reparametrization of a, b, c), check_s2 1854-1896, smooth_angle 1905-2085,
 diff_s_prep + check_signal 2091-2163 (legacy_policy.h:9-13), copy2s0 2169-2240.
 The force is delta_k * dE/dx_k from the FP64 analytic gradient (oracle
 pose_grad) by the chain rule, where the reference takes forward differences.
+
+Pinned by tests/test_string_oracle.py: the energy equals the sum of the
+oracle's triclinic pair (itself bit-equal to the reference header) over sites
+and atoms, and the gradient matches its central differences; smoothing
+against its closed form and the 0 / 2 pi wrap branches; after a step the
+endpoints are fixed, the last image is the periodic replica of the first and
+the chords are equal; exactly the images whose energy rose are reverted; a
+converged string stops only on a check iteration (planted energy functions).
 """
 from __future__ import annotations
 

@@ -9,12 +9,21 @@
  * rounded separately, as in the reference build (scripts/build_vext.sh:11-12,
  * g++ -O3 for baseline x86-64, which has no FMA).
  *
- * Pinning (see tests/test_oracle.py):
+ * Pinning (tests/test_oracle.py, tests/test_oracle_pins.py); no function here
+ * is left unpinned:
  *   oracle_reuse  == reference kernel.cpp compiled from source (oracle/_ref),
- *                    bit for bit, and == a brute-force all-images sum;
+ *                    bit for bit, and == a brute-force all-images sum; a
+ *                    planted dropped shift term fails;
  *   oracle_site   == the DIRECT formula checked against oracle_reuse at the
  *                    tolerances of tests/unit/test_public_api.py:60-61;
- *   oracle_pose_grad == central differences of oracle_pose_energy.
+ *   oracle_pose_grad == central differences of its own energy, which equals
+ *                    oracle_reuse;
+ *   oracle_string_pair == the reference's triclinic_pair.h through its own
+ *                    host shim (oracle/_ref), bit for bit, and the brute-force
+ *                    invariants of test_native_string_triclinic.py:59-90;
+ *   oracle_count_pairs == a brute-force count of d^2 < rc^2 over an oversized
+ *                    block of images; a pair exactly at the cutoff is not
+ *                    counted (a planted <= would count it).
  */
 #include <math.h>
 #include <stdint.h>

@@ -25,7 +25,19 @@ How each part is pinned (tests/test_oracle.py):
     triclinic_pair.h through its own host shim) and the brute force of
     test_native_string_triclinic.py:41-49;
   * ``pose_grad`` against central finite differences of ``pose_grad``'s own
-    energy (gradient parity), the energy itself against ``oracle_reuse``.
+    energy (gradient parity), the energy itself against ``oracle_reuse``;
+  * ``parse_string_input`` (tests/test_oracle_pins.py): the literal values of
+    the reference's toy ``input.dat`` and of tests/cof/examples/c2h4/input.dat
+    (kept under tests/golden/), the cell KAT through the parser, a planted
+    column swap must fail;
+  * ``body_offsets``: closed form for site masses 10 and 20, and
+    sum(m_s o_s) = 0, o_1 - o_0 = x_1 - x_0 for random unequal masses;
+  * ``count_pairs`` (the metric's numerator): a brute-force count over an
+    oversized image block, and a pair exactly at the cutoff not counted;
+  * ``marginal_energy``: closed forms (degenerate and two-level);
+  * ``error_metrics``: the accessible mask and the 1 K floor on constructed
+    inputs.
+No function here is left unpinned.
 """
 from __future__ import annotations
 
