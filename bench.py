@@ -20,13 +20,15 @@ same workload on this host's cores.
 The default line also carries summaries of the other workloads, each with its
 pair census, time per step, roofline fraction and error against FP64:
 "ragged" (100 frameworks x 2 guests in one launch), "poses" (8 x 65,536
-differentiable poses, forward + backward site kernels) and "strings" (64
-strings x 96 images, 2000 HIP-graph iterations).  --workload NAME prints one
-workload's full line instead.
+differentiable poses, forward + backward site kernels), "strings" (64
+strings x 96 images, 2000 HIP-graph iterations) and "e2e" (the string-vext
+CLI from input.dat to the NPZ and its provenance JSON).  --workload NAME
+prints one workload's full line instead.
 """
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -602,6 +604,51 @@ def dropin_line(args):
             "max_relative_vs_batched_fp64_floor1K": rel}
 
 
+def e2e_line(args):
+    """The `string-vext` CLI end to end (reference src/string_vext/vext.py:46-80):
+    parse input.dat, the 60^3 x 64 field with its marginal, grid and site
+    channels, the compressed NPZ and the provenance JSON with SHA256 digests,
+    for the 1200-atom framework written as an input.dat.  The reference
+    publishes 22.263 s for two gases (about 11 s per field) with its CPU kernel
+    on a 1568-atom framework (tests/cof/validation/vext_endtoend_speed_2materials.json)."""
+    import tempfile
+    from string_vext_amd import synthetic, vext
+    from string_vext_amd.input import parse_string_input, write_string_input
+    case = synthetic.full_field_case()
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        src = Path(tmp) / "input.dat"
+        write_string_input(case, src)
+        # warm-up: device plan setup, kernels loaded
+        vext.calculate_vext(parse_string_input(src), size=8, orientations=ORIENTATIONS)
+        for prec in ("fp64", "mixed"):
+            walls = []
+            for rep in range(max(1, args.steps)):
+                dst = Path(tmp) / f"vext_{prec}_{rep}.npz"
+                t = time.perf_counter()
+                # the CLI prints its provenance record; keep stdout for the one bench line
+                with contextlib.redirect_stdout(sys.stderr):
+                    vext.main(["--input", str(src), "--output", str(dst), "--size", str(SIZE),
+                               "--orientations", str(ORIENTATIONS), "--precision", prec])
+                walls.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            fields = vext.calculate_vext(parse_string_input(src), SIZE, ORIENTATIONS, precision=prec)
+            calc = time.perf_counter() - t
+            t = time.perf_counter()
+            with open(Path(tmp) / "probe.npz", "wb") as handle:
+                np.savez_compressed(handle, **fields)
+            npz = time.perf_counter() - t
+            out[prec] = {"cli_s": float(np.mean(walls)), "calculate_vext_s": calc,
+                         "npz_compress_write_s": npz}
+    return {"metric": "seconds per string-vext call, 60^3 x 64 field, input.dat to NPZ + JSON",
+            "value": out["mixed"]["cli_s"], "unit": "s", "n_gpus": 1, "steps": max(1, args.steps),
+            "ms_per_step": out["mixed"]["cli_s"] * 1e3, "higher_is_better": False,
+            "data": "synthetic", "config": {"workload": "string-vext CLI, 1200-atom framework "
+                                                        "(37x37x14 A), --size 60 --orientations 64"},
+            "by_precision": out,
+            "reference_cpu_s_per_field": 22.263 / 2}
+
+
 def compact(line, keep):
     """The headline line's summary of another workload's line."""
     out = {k: line[k] for k in ("metric", "value", "unit", "ms_per_step", "steps") if k in line}
@@ -717,11 +764,12 @@ def field_line(args):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", choices=("field", "ragged", "poses", "strings", "dropin"),
+    ap.add_argument("--workload", choices=("field", "ragged", "poses", "strings", "dropin", "e2e"),
                     default="field",
                     help="field (default): the 60^3 x 64 headline, with summaries of the ragged, "
-                         "poses and strings workloads; the others print one workload's full line "
-                         "(dropin: the reference's call sequence through the C ABI)")
+                         "poses, strings and e2e workloads; the others print one workload's full "
+                         "line (dropin: the reference's call sequence through the C ABI; e2e: the "
+                         "string-vext CLI from input.dat to NPZ)")
     ap.add_argument("--iterations", type=int, default=2000, help="strings workload iterations")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -749,7 +797,7 @@ def main():
         sys.exit("--steps must be >= 1")
     if args.workload != "field":
         line = {"ragged": ragged_line, "poses": poses_line, "strings": strings_line,
-                "dropin": dropin_line}[args.workload](args)
+                "dropin": dropin_line, "e2e": e2e_line}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
     line = field_line(args)
@@ -769,6 +817,9 @@ def main():
             k: {kk: vv for kk, vv in v.items() if not kk.startswith("barriers_")}
             for k, v in sl["frameworks"].items()}
         log(f"strings: {line['strings']['value']:.1f} iterations/s")
+        sub.steps = 1
+        line["e2e"] = compact(e2e_line(sub), ("config", "by_precision", "reference_cpu_s_per_field"))
+        log(f"e2e: {line['e2e']['value']:.2f} s per string-vext call (mixed)")
     print(json.dumps(line), flush=True)
 
 
