@@ -101,13 +101,16 @@ def _sha256(path: Path) -> str:
 def _publish_npz(destination: Path, fields: dict) -> None:
     """Write the NPZ under a unique temporary name in the destination's folder
     and hard-link it to its final name: readers never see a partial file, and
-    an existing destination makes the link (and the call) fail."""
+    an existing destination makes the link (and the call) fail.  The archive
+    is the format of numpy.savez_compressed, deflated in parallel chunks
+    (string_vext_amd.npz)."""
+    from .npz import write_npz
     destination.parent.mkdir(parents=True, exist_ok=True)
     fd, staging = tempfile.mkstemp(prefix=destination.name + ".", suffix=".partial",
                                    dir=destination.parent)
     try:
         with os.fdopen(fd, "wb") as handle:
-            np.savez_compressed(handle, **fields)
+            write_npz(handle, fields)
         os.link(staging, destination)
     finally:
         os.unlink(staging)
