@@ -16,7 +16,8 @@ name, and writes a provenance JSON beside it.  With the default kernel
 (libsv_vext.so) every channel comes from one fused GPU evaluation; with a
 library that only exports ``evaluate_reused_neighbors`` the energies go
 through that C ABI in 512-center blocks.  Added here: ``--precision mixed``
-(FP32 pair arithmetic, FP64 accumulation) and ``--near-energy``.
+(FP32 pair arithmetic, FP64 accumulation), ``--precision fp64_rcp`` (FP64
+with the hardware reciprocal) and ``--near-energy``.
 """
 from __future__ import annotations
 
@@ -73,6 +74,9 @@ def calculate_vext(case: dict, size: int = 60, orientations: int = 64,
         near = DEFAULT_NEAR_ENERGY_K if near_energy_K is None else near_energy_K
         orientation, marginal, grid, site = _gpu_channels(case, size, axes, precision, near)
     else:
+        if precision != "fp64":
+            raise ValueError(f"precision {precision!r} needs a kernel library with the batched "
+                             "entry points (sv_plan_create); this one only evaluates FP64")
         orientation, marginal, grid, site = _abi_channels(case, centers, axes, kernel)
     volume = (size, size, size)
     fields = {
@@ -138,7 +142,9 @@ def _arguments(argv):
     cli.add_argument("--orientations", type=int, default=64, help="Fibonacci directors")
     cli.add_argument("--kernel", type=Path, default=DEFAULT_KERNEL,
                      help="shared library exporting evaluate_reused_neighbors")
-    cli.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    cli.add_argument("--precision", choices=("fp64", "fp64_rcp", "mixed"), default="fp64",
+                     help="fp64: the reference's arithmetic; fp64_rcp: FP64 with the hardware "
+                          "reciprocal and one Newton step; mixed: FP32 pair terms, FP64 sums")
     cli.add_argument("--near-energy", type=float, default=None,
                      help="mixed precision: pair terms that can exceed this (K) run in FP64")
     return cli.parse_args(argv)

@@ -53,3 +53,18 @@ def test_npz_is_published_whole_and_never_overwritten(tmp_path):
     with np.load(out) as z:
         assert z["a"].tolist() == [0.0, 1.0, 2.0]
     assert sorted(p.name for p in out.parent.iterdir()) == ["vext.npz"]   # no staging files left
+
+
+def test_non_fp64_precision_needs_the_batched_library():
+    """A --kernel library that only exports evaluate_reused_neighbors (here the
+    reference kernel.cpp built from source) evaluates FP64; asking it for
+    another precision is refused before any evaluation."""
+    kernel = ROOT / "oracle" / "_ref" / "kernel.so"
+    if not kernel.exists():
+        pytest.skip("oracle/_ref/kernel.so is built only where the reference tree is present")
+    from string_vext_amd import synthetic
+    from string_vext_amd.vext import calculate_vext
+    case = synthetic.toy_case()
+    for precision in ("mixed", "fp64_rcp"):
+        with pytest.raises(ValueError, match="batched entry points"):
+            calculate_vext(case, size=2, orientations=4, kernel=kernel, precision=precision)
