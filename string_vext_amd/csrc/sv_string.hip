@@ -33,6 +33,7 @@
 // stride over its bin's list.  One iteration is four kernels (five in a
 // mixed run, with the FP64 recheck); the runtime captures them in a HIP graph
 // and replays it.
+#include "sv_device.h"
 #include "sv_objects.h"
 #include "sv_vext.h"
 
@@ -65,17 +66,6 @@ struct StringParams {
   double conv_o;   // orientation threshold
 };
 
-__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-__device__ __forceinline__ void wave_barrier_s() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ int lanes_below(unsigned long long m) {
-  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
 
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
@@ -148,7 +138,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
   __shared__ float4 s_cf[4][kStreamTypes];
   __shared__ double s_c6x[4][kStreamTypes];
   const int lane = threadIdx.x & 63;
-  const int wv = uniform_i(threadIdx.x >> 6);
+  const int wv = uniform(threadIdx.x >> 6);
   const int slot = blockIdx.x * 4 + wv;   // output index
   int p = slot;                           // image index
   const double* __restrict__ x = x_in;
@@ -170,7 +160,8 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
   double sb, cb, sg, cg;
   sincos(be, &sb, &cb);
   sincos(ga, &sg, &cg);
-  const double u[3] = {cg * cb, sg * cb, -sb};
+  // the image's director and center are wave-uniform: scalar registers
+  const double u[3] = {uniform_d(cg * cb), uniform_d(sg * cb), uniform_d(-sb)};
   const float uf[3] = {(float)u[0], (float)u[1], (float)u[2]};
   // bin of the center and its offset from the bin centre (fractional differences)
   int idx[3];
@@ -185,7 +176,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
   }
   double c[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) c[k] = df[0] * J.H[k] + df[1] * J.H[3 + k] + df[2] * J.H[6 + k];
+  for (int k = 0; k < 3; ++k) c[k] = uniform_d(df[0] * J.H[k] + df[1] * J.H[3 + k] + df[2] * J.H[6 + k]);
   const float cf[3] = {(float)c[0], (float)c[1], (float)c[2]};
   const long long seg0 = (long long)((idx[0] * IG.m[1] + idx[1]) * IG.m[2] + idx[2]) * J.n_types;
   const double o[2] = {J.o0, J.o1};
@@ -232,7 +223,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
       s_c6x[wv][lane] = J.c6x[lane];
     }
     if (lane <= n_types) s_bound[wv][lane] = IG.offsets[seg0 + lane];
-    wave_barrier_s();
+    wave_sync();
     const long long beg = s_bound[wv][0], end = s_bound[wv][n_types];
     const float sc = J.inv_rc2f;
     const float ox[2] = {J.o0sxf, J.o1sxf}, mx[2] = {J.m2o0xf, J.m2o1xf};
@@ -296,7 +287,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
         }
       }
       if (nq >= 64 || b0 + 64 >= end) {
-        wave_barrier_s();
+        wave_sync();
         for (int q0 = 0; q0 < nq; q0 += 64)
           if (q0 + lane < nq) {
             const long long id = beg + s_near[wv][q0 + lane];
@@ -306,7 +297,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
             const double qv[3] = {IG.list64[3 * id], IG.list64[3 * id + 1], IG.list64[3 * id + 2]};
             pair64r(qv, co.x, co.y, co.z, co.w);
           }
-        wave_barrier_s();
+        wave_sync();
         nq = 0;
       }
     }
@@ -320,7 +311,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
     if (lane < n_types)
       s_coef[wv][lane] = make_double4(J.c12[lane], J.c6[lane], J.shift[lane], J.floor2[lane]);
     if (lane <= n_types) s_bound[wv][lane] = IG.offsets[seg0 + lane];
-    wave_barrier_s();
+    wave_sync();
     const long long end = s_bound[wv][n_types];
     long long i = s_bound[wv][0] + lane;
     int ty = 0;
@@ -414,14 +405,14 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
         }
       }
       if (nq >= 64 || b0 + 64 >= end) {
-        wave_barrier_s();
+        wave_sync();
         for (int q0 = 0; q0 < nq; q0 += 64)
           if (q0 + lane < nq) {
             const long long id = beg + s_near[wv][q0 + lane];
             const double qv[3] = {IG.list64[3 * id], IG.list64[3 * id + 1], IG.list64[3 * id + 2]};
             pair64r(qv, c12, c6, sh, fl2);
           }
-        wave_barrier_s();
+        wave_sync();
         nq = 0;
       }
     }
