@@ -63,6 +63,24 @@ def test_dropin_abi_matches_oracle_on_small_field(oracle, lib, small_case):
     assert np.array_equal(out, ref)
 
 
+@pytest.mark.parametrize("n_axes", [1, 100])
+def test_dropin_abi_axis_counts_off_the_wave_width(oracle, lib, small_case, n_axes):
+    """One director, and 100 directors (a full group of 64 lanes and a
+    partial one): the drop-in call still reproduces the reference loop bit
+    for bit, with every output row written."""
+    setup = oracle.ReuseSetup(small_case)
+    rng = np.random.default_rng(20261204 + n_axes)
+    centers = rng.random((37, 3))
+    axes = rng.normal(size=(n_axes, 3))
+    axes /= np.linalg.norm(axes, axis=1, keepdims=True)
+    cart = np.mod(centers, 1) @ setup.cell
+    starts, indices = oracle.csr(setup.tree, cart, setup.radius)
+    rc, out = _legacy(lib, setup, cart, axes, starts, indices)
+    assert rc == 0 and np.isfinite(out).all()
+    ref = oracle.run_reuse_kernel(oracle.lib().oracle_reuse, cart, axes, setup, starts, indices)
+    assert np.array_equal(out, ref)
+
+
 def test_dropin_abi_error_codes(oracle, lib):
     case, centers, axes, ref = _golden.load("toy")
     setup = oracle.ReuseSetup(case)
