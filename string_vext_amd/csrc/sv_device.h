@@ -29,6 +29,15 @@ __device__ __forceinline__ double uniform_d(double v) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// p[i] read through the constant address space, for data no kernel writes
+// while it runs (plan and pose tables): with a wave-uniform address the
+// compiler issues a scalar load through the scalar cache instead of a vector
+// load and a readfirstlane.
+template <class T>
+__device__ __forceinline__ T ld_scalar(const T* p, long long i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8).
 // Map hardware block b to a logical block so that runs of kXcdRun consecutive
 // logical blocks (same tile, same candidate list) execute on one XCD and share

@@ -348,7 +348,7 @@ __device__ __forceinline__ void setup_item(const Job* __restrict__ jobs, int n_j
   const int ng = c.J->n_groups;
   c.slot = uniform(ng == 1 ? (int)local : (int)(local / ng));
   c.group = uniform((int)(local - (long long)c.slot * ng));
-  c.tile = uniform(c.J->tile_of[c.slot]);
+  c.tile = ld_scalar(c.J->tile_of, c.slot);
   const int lane = threadIdx.x & 63;
   c.k = c.group * 64 + lane;
   c.k_ok = c.k < c.J->n_axes;
@@ -397,7 +397,7 @@ __device__ __forceinline__ long long next_item(unsigned long long* __restrict__ 
 template <bool FAST = false>
 __device__ __forceinline__ void write_pose(const WaveCtx& c, double E, bool with_site, double site) {
   const Job& J = *c.J;
-  const int cid = J.center_id[c.slot];
+  const int cid = ld_scalar(J.center_id, c.slot);
   if (c.k_ok) {
     if (J.out64) J.out64[(long long)cid * J.n_axes + c.k] = E;
     if (J.out32) J.out32[(long long)c.k * J.n_centers + cid] = (float)E;
@@ -510,8 +510,9 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   const bool scale_u = SYM && k4d > 0.0;
   const double usc = SYM ? k4d : 1.0;
   const float ux = (float)(ux64 * usc), uy = (float)(uy64 * usc), uz = (float)(uz64 * usc);
-  const double cx = uniform_d(J.center_rel[3 * c.slot]), cy = uniform_d(J.center_rel[3 * c.slot + 1]),
-               cz = uniform_d(J.center_rel[3 * c.slot + 2]);
+  const double cx = ld_scalar(J.center_rel, 3LL * c.slot),
+               cy = ld_scalar(J.center_rel, 3LL * c.slot + 1),
+               cz = ld_scalar(J.center_rel, 3LL * c.slot + 2);
   const float cxf = uniform_f((float)cx), cyf = uniform_f((float)cy), czf = uniform_f((float)cz);
   const float rc2f = J.rc2f, keep2f = J.keep2f, m2o0 = J.m2o0f, m2o1 = J.m2o1f;
   const float o0sq = J.o0sqf, o1sq = J.o1sqf, in2 = J.in2f;
@@ -536,16 +537,16 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   double E = 0.0, S = 0.0, S_shift = 0.0;
   for (int t = 0; t < J.n_types; ++t) {
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
-    const float s6 = uniform_f(J.s6f[t]), fl2f = uniform_f(J.floor2f[t]);
-    const float near2 = uniform_f(J.near2f[t]), lo2 = uniform_f(J.lo2f[t]);
-    const double c6 = uniform_d(J.c6[t]), c12 = uniform_d(J.c12[t]), sh = uniform_d(J.shift[t]);
-    const double fl2 = uniform_d(J.floor2[t]);
+    const float s6 = ld_scalar(J.s6f, t), fl2f = ld_scalar(J.floor2f, t);
+    const float near2 = ld_scalar(J.near2f, t), lo2 = ld_scalar(J.lo2f, t);
+    const double c6 = ld_scalar(J.c6, t), c12 = ld_scalar(J.c12, t), sh = ld_scalar(J.shift, t);
+    const double fl2 = ld_scalar(J.floor2, t);
     const double m2c12 = uniform_d(-2.0 * c12);
     const double lo2d = (double)lo2;
-    const double c6s = uniform_d(J.c6s[t]);   // shell loop scale
+    const double c6s = ld_scalar(J.c6s, t);   // shell loop scale
     const float k128s6 = uniform_f(128.0f * s6);
     const double c6s6 = uniform_d(c6 * (double)s6);
-    const float s6s = uniform_f(J.s6sf[t]);
+    const float s6s = ld_scalar(J.s6sf, t);
     long long kept_far = 0, kept_out = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
     // Near-contact atoms are queued by list offset and evaluated when the queue
@@ -845,14 +846,16 @@ __global__ __launch_bounds__(256) void field_fp64_kernel(const Job* __restrict__
   const Job& J = *c.J;
   const bool do_site = J.site32 != nullptr && c.group == 0;
   const double ux = J.axes64[3 * c.k], uy = J.axes64[3 * c.k + 1], uz = J.axes64[3 * c.k + 2];
-  const double cx = J.center_rel[3 * c.slot], cy = J.center_rel[3 * c.slot + 1],
-               cz = J.center_rel[3 * c.slot + 2];
+  const double cx = ld_scalar(J.center_rel, 3LL * c.slot),
+               cy = ld_scalar(J.center_rel, 3LL * c.slot + 1),
+               cz = ld_scalar(J.center_rel, 3LL * c.slot + 2);
   const double keep2 = J.keep_r * J.keep_r, rc2 = J.rc2, o0 = J.o0, o1 = J.o1;
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
   double E = 0.0, S = 0.0;
   for (int t = 0; t < J.n_types; ++t) {
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
-    const double c12 = J.c12[t], c6 = J.c6[t], sh = J.shift[t], fl2 = J.floor2[t];
+    const double c12 = ld_scalar(J.c12, t), c6 = ld_scalar(J.c6, t), sh = ld_scalar(J.shift, t),
+                 fl2 = ld_scalar(J.floor2, t);
     for (long long base = beg; base < end; base += 64) {
       const long long idx = base + lane;
       double rx = 0.0, ry = 0.0, rz = 0.0;
@@ -894,8 +897,9 @@ __global__ __launch_bounds__(256) void census_kernel(const Job* __restrict__ job
   if (!setup_wave(jobs, n_jobs, job_work, n_work, wv, c)) return;
   const Job& J = *c.J;
   const double ux = J.axes64[3 * c.k], uy = J.axes64[3 * c.k + 1], uz = J.axes64[3 * c.k + 2];
-  const double cx = J.center_rel[3 * c.slot], cy = J.center_rel[3 * c.slot + 1],
-               cz = J.center_rel[3 * c.slot + 2];
+  const double cx = ld_scalar(J.center_rel, 3LL * c.slot),
+               cy = ld_scalar(J.center_rel, 3LL * c.slot + 1),
+               cz = ld_scalar(J.center_rel, 3LL * c.slot + 2);
   const double keep2 = J.keep_r * J.keep_r, rc2 = J.rc2, o0 = J.o0, o1 = J.o1;
   const long long seg0 = J.seg_base + (long long)c.tile * J.n_types;
   const long long beg = offsets[seg0], end = offsets[seg0 + J.n_types];
