@@ -721,7 +721,7 @@ def field_line(args):
         cpu = cpu_baseline(case, centers, axes, args.cpu_sample)
         log(f"cpu: {cpu['value']:.3e} pair-evals/s")
         threads = max(1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-        cpu_all = cpu_baseline_threads(case, centers, axes, 4 * args.cpu_sample, threads)
+        cpu_all = cpu_baseline_threads(case, centers, axes, min(len(centers), 65536), threads)
         log(f"cpu, {threads} threads: {cpu_all['value']:.3e} pair-evals/s")
         small = small_field_report(threads)
         log(f"20^3 field: {small}")
@@ -782,8 +782,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=5)
-    ap.add_argument("--cpu-sample", type=int, default=16384,
-                    help="grid centers in the CPU baseline sample (0 skips it)")
+    ap.add_argument("--cpu-sample", type=int, default=49152,
+                    help="grid centers in the single-core CPU baseline sample, about 11 s of "
+                         "reference-kernel time (0 skips the CPU baselines)")
     ap.add_argument("--no-check", action="store_true", help="skip the mixed-vs-FP64 error pass")
     ap.add_argument("--headline-only", action="store_true",
                     help="field workload: skip the ragged, poses and strings summaries")
