@@ -25,6 +25,16 @@ __device__ __forceinline__ double inv_fast(double x) {
 }
 
 
+// The per-type constants one mixed-field wave reads for each LJ type, in one
+// 64-byte record so that a type costs a single scalar block load.
+struct alignas(64) FieldType {
+  double c12, c6, shift, floor2;  // as the Job's per-type arrays
+  double c6s;                     // c6 / rc^6 (shell form)
+  float s6f, floor2f, near2f, lo2f, s6sf;
+  float pad;
+};
+static_assert(sizeof(FieldType) == 64, "one 64-byte record per type");
+
 // One field job = (framework, guest, pose set, axes, outputs).  A single
 // Vext field is a batch of one job; a ragged batch is many jobs launched
 // together.  Lives in device global memory and is read with uniform
@@ -70,6 +80,7 @@ struct Job {
                         // orientation reaches the contact floor
   const double* c6s;    // c6 / rc^6            (shell form of the mixed path)
   const float* s6sf;    // sigma^6 / rc^6 in FP32
+  const FieldType* ftype; // the mixed field's per-type constants, one record per type
   const double* c6x;    // c6 s^3               (pose form, s = inv_rc2f)
   const float* s6xf;    // sigma^6 s^3 in FP32
   // ---- pose set -----------------------------------------------------------

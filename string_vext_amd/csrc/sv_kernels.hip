@@ -486,6 +486,16 @@ struct MixedLds {
   int near_idx[4][64]; // queued near-contact list entries (offset in the type segment)
 };
 
+// Type t's 64-byte record through scalar loads (eight 8-byte words).
+__device__ __forceinline__ FieldType ld_field_type(const FieldType* p, int t) {
+  unsigned long long w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = ld_scalar(reinterpret_cast<const unsigned long long*>(p + t), i);
+  FieldType f;
+  __builtin_memcpy(&f, w, sizeof(f));
+  return f;
+}
+
 // One pose center x 64 orientations of the mixed-precision field.
 template <bool SYM>
 __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L,
@@ -537,16 +547,17 @@ __device__ __forceinline__ void mixed_item(const WaveCtx& c, int wv, MixedLds& L
   double E = 0.0, S = 0.0, S_shift = 0.0;
   for (int t = 0; t < J.n_types; ++t) {
     const long long beg = offsets[seg0 + t], end = offsets[seg0 + t + 1];
-    const float s6 = ld_scalar(J.s6f, t), fl2f = ld_scalar(J.floor2f, t);
-    const float near2 = ld_scalar(J.near2f, t), lo2 = ld_scalar(J.lo2f, t);
-    const double c6 = ld_scalar(J.c6, t), c12 = ld_scalar(J.c12, t), sh = ld_scalar(J.shift, t);
-    const double fl2 = ld_scalar(J.floor2, t);
+    const FieldType ft = ld_field_type(J.ftype, t);
+    const float s6 = ft.s6f, fl2f = ft.floor2f;
+    const float near2 = ft.near2f, lo2 = ft.lo2f;
+    const double c6 = ft.c6, c12 = ft.c12, sh = ft.shift;
+    const double fl2 = ft.floor2;
     const double m2c12 = uniform_d(-2.0 * c12);
     const double lo2d = (double)lo2;
-    const double c6s = ld_scalar(J.c6s, t);   // shell loop scale
+    const double c6s = ft.c6s;   // shell loop scale
     const float k128s6 = uniform_f(128.0f * s6);
     const double c6s6 = uniform_d(c6 * (double)s6);
-    const float s6s = ld_scalar(J.s6sf, t);
+    const float s6s = ft.s6sf;
     long long kept_far = 0, kept_out = 0;
     float Sf = 0.0f;   // this lane's FP32 site terms of this type (one per chunk)
     // Near-contact atoms are queued by list offset and evaluated when the queue

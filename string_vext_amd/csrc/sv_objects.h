@@ -55,6 +55,7 @@ struct Plan {
   DevBuf<double> d_frac, d_c12, d_c6, d_shift, d_floor2, d_sigma, d_eps, d_c6s, d_c6x;
   DevBuf<int> d_type_start;
   DevBuf<float> d_s6f, d_floor2f, d_near2f, d_lo2f, d_s6sf, d_s6xf, d_nsite2f;
+  DevBuf<FieldType> d_ftype;
   ~Plan() {
     if (stream) (void)hipStreamDestroy(stream);
   }

@@ -111,6 +111,7 @@ void plan_fill_job(const Plan& P, Job& J) {
   J.lo2f = P.d_lo2f.p;
   J.c6s = P.d_c6s.p;
   J.s6sf = P.d_s6sf.p;
+  J.ftype = P.d_ftype.p;
   J.c6x = P.d_c6x.p;
   J.s6xf = P.d_s6xf.p;
   J.o0sxf = (float)(P.o0 * P.o0 * (double)(float)(1.0 / J.rc2));
@@ -712,8 +713,13 @@ int plan_create(const double* cell, int n_atoms, const double* frac, const doubl
     const double r = std::sqrt((double)std::max(P->near2f[t], P->lo2f[t]));
     if (!(r * (1.0 + 1e-6) + omax < cutoff)) P->near_inside = false;
   }
+  std::vector<FieldType> ftype(P->n_types);
+  for (int t = 0; t < P->n_types; ++t)
+    ftype[t] = FieldType{P->c12[t], P->c6[t], P->shift[t], P->floor2[t], P->c6s[t], P->s6f[t],
+                         P->floor2f[t], P->near2f[t], P->lo2f[t], P->s6sf[t], 0.0f};
   hipError_t e;
   if ((e = hipGetDevice(&P->device)) != hipSuccess || (e = hipStreamCreate(&P->stream)) != hipSuccess ||
+      (e = P->d_ftype.upload(ftype.data(), ftype.size(), P->stream)) != hipSuccess ||
       (e = P->d_frac.upload(P->frac.data(), P->frac.size(), P->stream)) != hipSuccess ||
       (e = P->d_type_start.upload(P->type_start.data(), P->type_start.size(), P->stream)) != hipSuccess ||
       (e = P->d_c12.upload(P->c12.data(), P->n_types, P->stream)) != hipSuccess ||
