@@ -12,14 +12,18 @@ enqueued on torch's current stream without a host synchronization
 operations, and ``check_poses(plan)`` raises on them.
 
 Two ways to the gradient:
-  * default: forward runs the energy kernel; backward runs a separate kernel
-    that recomputes every pair term and returns dE/dc and dE/du scaled by the
-    upstream gradient.  When no other evaluation has used the grid since this
-    forward, backward reuses the forward's binning (sv_pose_grid_reeval_async);
-  * ``fused=True``: when an input requires a gradient, forward runs one kernel
-    that sums each site's energy and gradient together and keeps the pose
-    gradients dE/dc, dE/du (6 doubles per pose); backward only scales them by
-    the upstream gradient.  One pass over the pair terms instead of two.
+  * fused (the default when gradients are being recorded, i.e. grad mode is on
+    and centers or axes require a gradient): forward runs one kernel that sums
+    each site's energy and gradient together and keeps the pose gradients
+    dE/dc, dE/du (6 doubles per pose); backward only scales them by the
+    upstream gradient.  One pass over the pair terms instead of two; a forward
+    whose backward never runs has paid about a third more than the energy alone;
+  * two-pass (``fused=False``, and whenever nothing is recorded): forward runs
+    the energy kernel; backward runs a separate kernel that recomputes every
+    pair term and returns dE/dc and dE/du scaled by the upstream gradient.
+    When no other evaluation has used the grid since this forward, backward
+    reuses the forward's binning (sv_pose_grid_reeval_async).
+Both give the same energies and gradients bit for bit.
 
 There is no counterpart in the reference, whose String forces are forward
 differences of the energy (native/string/source_code/global.h:1281-1324).
@@ -104,6 +108,11 @@ def check_poses(plan: Plan) -> None:
 
 
 def pose_energy(plan: Plan, centers: torch.Tensor, axes: torch.Tensor,
-                precision: str = "fp64", fused: bool = False) -> torch.Tensor:
-    """Differentiable energies of independent poses (center_i, director_i)."""
+                precision: str = "fp64", fused: bool | None = None) -> torch.Tensor:
+    """Differentiable energies of independent poses (center_i, director_i).
+
+    ``fused`` None (default) takes the fused route exactly when autograd is
+    recording the call; True or False forces one route."""
+    if fused is None:
+        fused = torch.is_grad_enabled() and (centers.requires_grad or axes.requires_grad)
     return PoseEnergy.apply(centers, axes, plan, precision, fused)

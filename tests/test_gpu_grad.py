@@ -343,17 +343,17 @@ def test_pose_grid_reeval_reuses_binning(case, poses):
         grid.check()
         c.copy_(c_saved)
         assert torch.equal(got[0], ref[0]) and torch.equal(got[1], ref[1])
-    # autograd: forward A, forward B on the same grid, then backward of A
+    # autograd, two-pass route: forward A, forward B on the same grid, then backward of A
     cg = torch.tensor(centers, dtype=torch.float64, device="cuda", requires_grad=True)
     ug = torch.tensor(axes, dtype=torch.float64, device="cuda", requires_grad=True)
-    ea = pose_energy(plan, cg, ug, "fp64")
-    eb = pose_energy(plan, cg.detach().flip(0), ug.detach().flip(0), "fp64")
+    ea = pose_energy(plan, cg, ug, "fp64", fused=False)
+    eb = pose_energy(plan, cg.detach().flip(0), ug.detach().flip(0), "fp64", fused=False)
     assert _grid_for(plan, n).binned_token is not None
     ea.sum().backward()
     gc_a, gu_a = cg.grad.clone(), ug.grad.clone()
     cg.grad = None
     ug.grad = None
-    pose_energy(plan, cg, ug, "fp64").sum().backward()      # reuses the binning
+    pose_energy(plan, cg, ug, "fp64", fused=False).sum().backward()      # reuses the binning
     assert torch.equal(cg.grad, gc_a) and torch.equal(ug.grad, gu_a)
     assert torch.equal(eb.flip(0), ea.detach())
 

@@ -137,3 +137,23 @@ def test_candidate_csr_equals_ball_queries(oracle):
     np.testing.assert_array_equal(starts, ref_starts)
     for c in range(len(cart)):
         assert set(indices[starts[c]:starts[c + 1]]) == set(ref_indices[ref_starts[c]:ref_starts[c + 1]])
+
+
+def test_pose_energy_route_follows_autograd_recording(monkeypatch):
+    """pose_energy's default takes the fused route (gradient in the forward's
+    site pass) exactly when autograd records the call; an explicit flag wins."""
+    import torch
+    from string_vext_amd import autograd
+    seen = []
+    monkeypatch.setattr(autograd.PoseEnergy, "apply",
+                        lambda c, u, plan, precision, fused: seen.append(fused))
+    c = torch.zeros(2, 3, dtype=torch.float64, requires_grad=True)
+    u = torch.ones(2, 3, dtype=torch.float64)
+    autograd.pose_energy(None, c, u)
+    autograd.pose_energy(None, c.detach(), u.requires_grad_(True))
+    with torch.no_grad():
+        autograd.pose_energy(None, c, u)
+    autograd.pose_energy(None, c.detach(), u.detach())
+    autograd.pose_energy(None, c, u, fused=False)
+    autograd.pose_energy(None, c.detach(), u.detach(), fused=True)
+    assert seen == [True, True, False, False, False, True]
