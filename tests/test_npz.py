@@ -60,14 +60,35 @@ def test_zip64_records_when_forced(monkeypatch):
     _check(buf, fields)
 
 
-def test_chunks_form_one_deflate_stream():
+@pytest.mark.parametrize("strategy", ["rle", "default"])
+def test_chunks_form_one_deflate_stream(strategy):
+    """Chunks joined by sync flushes decompress as one stream; with LZ77 string
+    matches (numpy's strategy) the back-references across a chunk boundary go
+    through the preset dictionary."""
     import zlib
     from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(3)
     data = memoryview(np.repeat(rng.integers(0, 7, size=300_000, dtype=np.uint8), 11))
+    st = {"rle": zlib.Z_RLE, "default": zlib.Z_DEFAULT_STRATEGY}[strategy]
     with ThreadPoolExecutor(4) as pool:
-        stream = npz.deflate_parallel(data, pool)
+        stream = npz.deflate_parallel(data, pool, strategy=st)
     assert zlib.decompress(stream, -15) == bytes(data)
+
+
+def test_strategies_round_trip_and_runs_compress():
+    """Both strategies give archives numpy reads back; the default (runs only)
+    still compresses a long run to a small fraction of its size."""
+    import zlib
+    fields = {"zeros": np.zeros(1 << 20, dtype=np.float32),
+              "noise": np.random.default_rng(4).normal(size=50_000).astype(np.float32)}
+    sizes = {}
+    for st in (zlib.Z_RLE, zlib.Z_DEFAULT_STRATEGY, zlib.Z_HUFFMAN_ONLY):
+        buf = io.BytesIO()
+        npz.write_npz(buf, fields, threads=2, strategy=st)
+        _check(buf, fields)
+        sizes[st] = buf.tell()
+    assert sizes[zlib.Z_RLE] < 0.05 * (4 << 20)
+    assert sizes[zlib.Z_RLE] < sizes[zlib.Z_HUFFMAN_ONLY]
 
 
 def test_corrupted_member_fails_its_crc():
@@ -85,3 +106,20 @@ def test_corrupted_member_fails_its_crc():
 def test_object_arrays_refused():
     with pytest.raises(ValueError):
         npz.write_npz(io.BytesIO(), {"o": np.array([object()])})
+
+
+def test_crc32_combine_equals_whole_buffer_crc():
+    """The chunk CRCs combined over GF(2) give zlib's CRC-32 of the whole
+    buffer, for empty, short, chunk-sized and ragged parts."""
+    import zlib
+    rng = np.random.default_rng(5)
+    for la, lb in ((0, 0), (0, 7), (5, 0), (1, 1), (1000, 1 << 20), ((1 << 20) + 3, 12345)):
+        a = rng.integers(0, 256, la, dtype=np.uint8).tobytes()
+        b = rng.integers(0, 256, lb, dtype=np.uint8).tobytes()
+        assert npz.crc32_combine(zlib.crc32(a), zlib.crc32(b), lb) == zlib.crc32(a + b)
+    # a multi-chunk member: the stream's CRC is that of the whole buffer
+    from concurrent.futures import ThreadPoolExecutor
+    data = memoryview(rng.integers(0, 256, 3 * npz.CHUNK + 777, dtype=np.uint8))
+    with ThreadPoolExecutor(3) as pool:
+        stream, crc = npz.deflate_crc_parallel(data, pool)
+    assert crc == zlib.crc32(data) and zlib.decompress(stream, -15) == bytes(data)
