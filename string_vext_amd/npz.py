@@ -26,7 +26,9 @@ from __future__ import annotations
 
 import io
 import os
+import queue
 import struct
+import threading
 import time
 import zlib
 from concurrent.futures import ThreadPoolExecutor
@@ -104,19 +106,27 @@ def _deflate_chunk(buf: memoryview, lo: int, hi: int, last: bool, level: int,
     return body + comp.flush(zlib.Z_FINISH if last else zlib.Z_SYNC_FLUSH), zlib.crc32(buf[lo:hi])
 
 
-def deflate_crc_parallel(buf: memoryview, pool: ThreadPoolExecutor, level: int = LEVEL,
-                         strategy: int = STRATEGY):
-    """One raw deflate stream of `buf` and its CRC-32, both formed chunk by
-    chunk on `pool`."""
+def _deflate_chunks(buf: memoryview, pool: ThreadPoolExecutor, level: int, strategy: int):
+    """The chunks of `buf`'s raw deflate stream in order, as (length of input,
+    compressed bytes, CRC-32 of the input), each as soon as it and the ones
+    before it are done."""
     n = len(buf)
     bounds = list(range(0, n, CHUNK)) or [0]
     jobs = [pool.submit(_deflate_chunk, buf, lo, min(lo + CHUNK, n), lo + CHUNK >= n, level,
                         strategy) for lo in bounds]
-    parts, crc = [], 0
     for lo, job in zip(bounds, jobs):
         body, c = job.result()
+        yield min(lo + CHUNK, n) - lo, body, c
+
+
+def deflate_crc_parallel(buf: memoryview, pool: ThreadPoolExecutor, level: int = LEVEL,
+                         strategy: int = STRATEGY):
+    """One raw deflate stream of `buf` and its CRC-32, both formed chunk by
+    chunk on `pool`."""
+    parts, crc = [], 0
+    for j, (length, body, c) in enumerate(_deflate_chunks(buf, pool, level, strategy)):
         parts.append(body)
-        crc = crc32_combine(crc, c, min(lo + CHUNK, n) - lo) if lo else c
+        crc = crc32_combine(crc, c, length) if j else c
     return b"".join(parts), crc
 
 
@@ -124,6 +134,32 @@ def deflate_parallel(buf: memoryview, pool: ThreadPoolExecutor, level: int = LEV
                      strategy: int = STRATEGY) -> bytes:
     """One raw deflate stream of `buf`, compressed chunk by chunk on `pool`."""
     return deflate_crc_parallel(buf, pool, level, strategy)[0]
+
+
+class _Digester:
+    """Feeds written blocks, in order, to a hashlib object on a thread of its
+    own (hashlib releases the GIL on large blocks), so that hashing overlaps
+    the compression of the blocks after them."""
+
+    def __init__(self, digest):
+        self.digest = digest
+        self.blocks = queue.Queue(maxsize=256)
+        self.thread = threading.Thread(target=self._run, daemon=True)
+        self.thread.start()
+
+    def _run(self):
+        while True:
+            block = self.blocks.get()
+            if block is None:
+                return
+            self.digest.update(block)
+
+    def feed(self, block: bytes):
+        self.blocks.put(block)
+
+    def close(self):
+        self.blocks.put(None)
+        self.thread.join()
 
 
 def _dos_time(t: float):
@@ -138,53 +174,80 @@ class _Member:
 
 
 def write_npz(handle, fields: dict, level: int = LEVEL, threads: int | None = None,
-              strategy: int = STRATEGY) -> None:
+              strategy: int = STRATEGY, digest=None) -> None:
     """Write `fields` (name -> array) to the binary file object `handle` as a
     compressed NPZ (members ``<name>.npy``, deflate level `level` with zlib
-    strategy `strategy`)."""
+    strategy `strategy`).
+
+    Each member is written as its chunks finish: its local header carries
+    zero sizes and CRC with general-purpose flag bit 3, and a data descriptor
+    after the data holds them (the central directory has them too, as zip
+    readers use).  With `digest` (a hashlib object), every byte written is
+    also fed to it, in order, so that the archive's hash is ready when the
+    archive is."""
     workers = threads or min(16, os.cpu_count() or 1)
     dtime, ddate = _dos_time(time.time())
+    flags = 0x0808   # bit 3: sizes and CRC in a data descriptor; bit 11: UTF-8 names
     members = []
     pos = 0
+    digester = _Digester(digest) if digest is not None else None
 
     def emit(data: bytes) -> None:
         nonlocal pos
         handle.write(data)
+        if digester is not None:
+            digester.feed(data)
         pos += len(data)
 
-    with ThreadPoolExecutor(max_workers=workers) as pool:
-        for key, value in fields.items():
-            name = (str(key) + ".npy").encode("utf-8")
-            raw = _npy_bytes(value)
-            comp, crc = deflate_crc_parallel(raw, pool, level, strategy)
-            m = _Member(name, crc & 0xFFFFFFFF, len(comp), len(raw), pos)
-            big = m.usize >= ZIP64_LIMIT or m.csize >= ZIP64_LIMIT
-            extra = struct.pack("<HHQQ", 0x0001, 16, m.usize, m.csize) if big else b""
-            emit(struct.pack("<IHHHHHIIIHH", 0x04034B50, 45 if big else 20, 0x0800, 8, dtime,
-                             ddate, m.crc, _MARK32 if big else m.csize,
-                             _MARK32 if big else m.usize, len(name), len(extra)) + name + extra)
-            emit(comp)
-            members.append(m)
-    cd_start = pos
-    for m in members:
-        sizes64 = m.usize >= ZIP64_LIMIT or m.csize >= ZIP64_LIMIT
-        offset64 = m.offset >= ZIP64_LIMIT
-        values = ([m.usize, m.csize] if sizes64 else []) + ([m.offset] if offset64 else [])
-        extra = (struct.pack(f"<HH{len(values)}Q", 0x0001, 8 * len(values), *values)
-                 if values else b"")
-        version = 45 if values else 20
-        # made by: Unix host (3), so that the external attributes are permissions
-        emit(struct.pack("<IHHHHHHIIIHHHHHII", 0x02014B50, (3 << 8) | version, version, 0x0800, 8,
-                         dtime, ddate, m.crc, _MARK32 if sizes64 else m.csize,
-                         _MARK32 if sizes64 else m.usize, len(m.name), len(extra), 0, 0, 0,
-                         0o644 << 16, _MARK32 if offset64 else m.offset) + m.name + extra)
-    cd_size = pos - cd_start
-    n = len(members)
-    zip64 = n >= ZIP64_COUNT or cd_start >= ZIP64_LIMIT or cd_size >= ZIP64_LIMIT
-    if zip64:
-        eocd64 = pos
-        emit(struct.pack("<IQHHIIQQQQ", 0x06064B50, 44, (3 << 8) | 45, 45, 0, 0, n, n, cd_size,
-                         cd_start))
-        emit(struct.pack("<IIQI", 0x07064B50, 0, eocd64, 1))
-    emit(struct.pack("<IHHHHIIH", 0x06054B50, 0, 0, 0xFFFF if zip64 else n, 0xFFFF if zip64 else n,
-                     _MARK32 if zip64 else cd_size, _MARK32 if zip64 else cd_start, 0))
+    try:
+        with ThreadPoolExecutor(max_workers=workers) as pool:
+            for key, value in fields.items():
+                name = (str(key) + ".npy").encode("utf-8")
+                raw = _npy_bytes(value)
+                usize = len(raw)
+                # zip64 sizes when the member or its deflate stream (which
+                # can exceed the input by a few bytes per block) may pass 4 GiB
+                big = usize + usize // 64 + 4096 >= ZIP64_LIMIT
+                extra = struct.pack("<HHQQ", 0x0001, 16, 0, 0) if big else b""
+                offset = pos
+                emit(struct.pack("<IHHHHHIIIHH", 0x04034B50, 45 if big else 20, flags, 8, dtime,
+                                 ddate, 0, _MARK32 if big else 0, _MARK32 if big else 0,
+                                 len(name), len(extra)) + name + extra)
+                csize = crc = 0
+                for j, (length, body, c) in enumerate(_deflate_chunks(raw, pool, level, strategy)):
+                    emit(body)
+                    csize += len(body)
+                    crc = crc32_combine(crc, c, length) if j else c
+                crc &= 0xFFFFFFFF
+                if not big and csize >= ZIP64_LIMIT:
+                    raise RuntimeError("deflate stream outgrew its member's 32-bit size fields")
+                emit(struct.pack("<IIQQ", 0x08074B50, crc, csize, usize) if big
+                     else struct.pack("<IIII", 0x08074B50, crc, csize, usize))
+                members.append(_Member(name, crc, csize, usize, offset))
+        cd_start = pos
+        for m in members:
+            sizes64 = m.usize >= ZIP64_LIMIT or m.csize >= ZIP64_LIMIT
+            offset64 = m.offset >= ZIP64_LIMIT
+            values = ([m.usize, m.csize] if sizes64 else []) + ([m.offset] if offset64 else [])
+            extra = (struct.pack(f"<HH{len(values)}Q", 0x0001, 8 * len(values), *values)
+                     if values else b"")
+            version = 45 if values else 20
+            # made by: Unix host (3), so that the external attributes are permissions
+            emit(struct.pack("<IHHHHHHIIIHHHHHII", 0x02014B50, (3 << 8) | version, version, flags,
+                             8, dtime, ddate, m.crc, _MARK32 if sizes64 else m.csize,
+                             _MARK32 if sizes64 else m.usize, len(m.name), len(extra), 0, 0, 0,
+                             0o644 << 16, _MARK32 if offset64 else m.offset) + m.name + extra)
+        cd_size = pos - cd_start
+        n = len(members)
+        zip64 = n >= ZIP64_COUNT or cd_start >= ZIP64_LIMIT or cd_size >= ZIP64_LIMIT
+        if zip64:
+            eocd64 = pos
+            emit(struct.pack("<IQHHIIQQQQ", 0x06064B50, 44, (3 << 8) | 45, 45, 0, 0, n, n, cd_size,
+                             cd_start))
+            emit(struct.pack("<IIQI", 0x07064B50, 0, eocd64, 1))
+        emit(struct.pack("<IHHHHIIH", 0x06054B50, 0, 0, 0xFFFF if zip64 else n,
+                         0xFFFF if zip64 else n, _MARK32 if zip64 else cd_size,
+                         _MARK32 if zip64 else cd_start, 0))
+    finally:
+        if digester is not None:
+            digester.close()

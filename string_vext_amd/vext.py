@@ -102,28 +102,32 @@ def _sha256(path: Path) -> str:
     return digest.hexdigest()
 
 
-def _publish_npz(destination: Path, fields: dict) -> None:
+def _publish_npz(destination: Path, fields: dict) -> str:
     """Write the NPZ under a unique temporary name in the destination's folder
     and hard-link it to its final name: readers never see a partial file, and
     an existing destination makes the link (and the call) fail.  The archive
     is the format of numpy.savez_compressed, deflated in parallel chunks
-    (string_vext_amd.npz)."""
+    (string_vext_amd.npz).  Returns the SHA-256 of the file, hashed from the
+    bytes as they are written."""
     from .npz import write_npz
     destination.parent.mkdir(parents=True, exist_ok=True)
     fd, staging = tempfile.mkstemp(prefix=destination.name + ".", suffix=".partial",
                                    dir=destination.parent)
+    digest = hashlib.sha256()
     try:
         with os.fdopen(fd, "wb") as handle:
-            write_npz(handle, fields)
+            write_npz(handle, fields, digest=digest)
         os.link(staging, destination)
     finally:
         os.unlink(staging)
+    return digest.hexdigest()
 
 
-def _provenance(args, case) -> dict:
+def _provenance(args, case, output_sha256: str | None = None) -> dict:
     sources = {"input": args.input, "output": args.output, "kernel": args.kernel}
     record = {"input": str(args.input.resolve())}
-    record.update({f"{name}_sha256": _sha256(path) for name, path in sources.items()})
+    record.update({f"{name}_sha256": output_sha256 if name == "output" and output_sha256
+                   else _sha256(path) for name, path in sources.items()})
     record.update(size=args.size, orientations=args.orientations, energy_mode=ENERGY_MODE,
                   precision=args.precision, cell_lengths_A=case["cell_lengths"],
                   cell_angles_deg=case["cell_angles_deg"],
@@ -156,9 +160,9 @@ def main(argv=None):
     if any(p.exists() for p in (args.output, record_path)):
         raise FileExistsError("Vext output or provenance record already exists")
     case = parse_string_input(args.input)
-    _publish_npz(args.output, calculate_vext(case, args.size, args.orientations, args.kernel,
-                                             args.precision, args.near_energy))
-    record = _provenance(args, case)
+    output_sha256 = _publish_npz(args.output, calculate_vext(
+        case, args.size, args.orientations, args.kernel, args.precision, args.near_energy))
+    record = _provenance(args, case, output_sha256)
     with record_path.open("x") as handle:
         handle.write(json.dumps(record, indent=2) + "\n")
     print(json.dumps(record, sort_keys=True))

@@ -56,6 +56,9 @@ def test_cli_writes_npz_and_provenance_once(case, tmp_path):
                 "energy_mode", "cell_lengths_A", "cell_angles_deg"):
         assert key in record
     assert record["size"] == 4 and record["precision"] == "mixed"
+    # hashed while written: the digest of the file as it lies on disk
+    import hashlib
+    assert record["output_sha256"] == hashlib.sha256(out.read_bytes()).hexdigest()
     with pytest.raises(FileExistsError):
         main(argv)
 

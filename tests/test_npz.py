@@ -123,3 +123,17 @@ def test_crc32_combine_equals_whole_buffer_crc():
     with ThreadPoolExecutor(3) as pool:
         stream, crc = npz.deflate_crc_parallel(data, pool)
     assert crc == zlib.crc32(data) and zlib.decompress(stream, -15) == bytes(data)
+
+
+def test_digest_is_the_hash_of_the_written_bytes(monkeypatch):
+    """write_npz(..., digest=h) feeds h every byte written, in order: its
+    digest equals the hash of the archive (plain and with zip64 records)."""
+    import hashlib
+    fields = _fields(np.random.default_rng(6))
+    for limit in (npz.ZIP64_LIMIT, 1000):
+        monkeypatch.setattr(npz, "ZIP64_LIMIT", limit)
+        buf = io.BytesIO()
+        h = hashlib.sha256()
+        npz.write_npz(buf, fields, threads=4, digest=h)
+        assert h.hexdigest() == hashlib.sha256(buf.getvalue()).hexdigest()
+        _check(buf, fields)

@@ -44,8 +44,10 @@ def test_console_script_refuses_an_existing_output(tmp_path):
 def test_npz_is_published_whole_and_never_overwritten(tmp_path):
     import numpy as np
     from string_vext_amd.vext import _publish_npz
+    import hashlib
     out = tmp_path / "sub" / "vext.npz"
-    _publish_npz(out, {"a": np.arange(3.0)})
+    digest = _publish_npz(out, {"a": np.arange(3.0)})
+    assert digest == hashlib.sha256(out.read_bytes()).hexdigest()
     with np.load(out) as z:
         assert z["a"].tolist() == [0.0, 1.0, 2.0]
     with pytest.raises(FileExistsError):
