@@ -23,7 +23,9 @@ Two ways to the gradient:
     pair term and returns dE/dc and dE/du scaled by the upstream gradient.
     When no other evaluation has used the grid since this forward, backward
     reuses the forward's binning (sv_pose_grid_reeval_async).
-Both give the same energies and gradients bit for bit.
+Both give the same energies and gradients bit for bit.  The kernels give
+first derivatives only: differentiating the gradient again (create_graph with
+a loss nonlinear in E) raises rather than dropping the Hessian term.
 
 There is no counterpart in the reference, whose String forces are forward
 differences of the energy (native/string/source_code/global.h:1281-1324).
@@ -31,6 +33,7 @@ differences of the energy (native/string/source_code/global.h:1281-1324).
 from __future__ import annotations
 
 import torch
+from torch.autograd.function import once_differentiable
 
 from .field import Plan, PoseGrid
 
@@ -77,6 +80,7 @@ class PoseEnergy(torch.autograd.Function):
         return energy
 
     @staticmethod
+    @once_differentiable   # the kernels give first derivatives only: a second raises
     def backward(ctx, grad_energy):
         if ctx.fused:
             grad_c, grad_u = ctx.saved_tensors

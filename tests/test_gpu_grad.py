@@ -400,3 +400,22 @@ def test_non_unit_directors_are_used_as_given(oracle, case, poses):
         for g, ref in ((gc, gc_ref), (gu, gu_ref)):
             scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)
             assert (np.abs(g - ref) / scale).max() < g_tol
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_second_derivative_is_refused(case, poses, fused):
+    """The kernels give first derivatives only: differentiating the gradient
+    again raises instead of returning zeros."""
+    import torch
+    from string_vext_amd.autograd import pose_energy
+    centers, axes = poses
+    plan = _plan(case)
+    c = torch.tensor(centers[:64], dtype=torch.float64, device="cuda", requires_grad=True)
+    u = torch.tensor(axes[:64], dtype=torch.float64, device="cuda")
+    e = pose_energy(plan, c, u, "fp64", fused=fused)
+    # a loss whose gradient in E depends on c: the second derivative would need
+    # the Hessian of E, which no kernel provides
+    (g,) = torch.autograd.grad((e * e).sum(), c, create_graph=True)
+    assert torch.isfinite(g).all()
+    with pytest.raises(RuntimeError, match="once_differentiable"):
+        g.sum().backward()
