@@ -63,6 +63,53 @@ def test_dropin_abi_matches_oracle_on_small_field(oracle, lib, small_case):
     assert np.array_equal(out, ref)
 
 
+def test_dropin_abi_calls_from_several_threads(oracle, lib, small_case):
+    """ctypes releases the GIL around the call, so several host threads may
+    call the drop-in symbol at once (the reference loads it the same way):
+    every concurrent call returns what the same call returns alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    setup = oracle.ReuseSetup(small_case)
+    centers, axes = oracle.regular_pose_grid(6, 64)
+    cart = np.mod(centers, 1) @ setup.cell
+    jobs = []
+    for k in range(4):
+        part = cart[54 * k:54 * (k + 1)]
+        starts, indices = oracle.csr(setup.tree, part, setup.radius)
+        jobs.append((part, starts, indices))
+    alone = [_legacy(lib, setup, c, axes, st, ix) for c, st, ix in jobs]
+
+    def repeat(job):
+        return [_legacy(lib, setup, job[0], axes, job[1], job[2]) for _ in range(3)]
+
+    with ThreadPoolExecutor(4) as pool:
+        together = list(pool.map(repeat, jobs))
+    for (rc0, out0), runs in zip(alone, together):
+        assert rc0 == 0
+        for rc, out in runs:
+            assert rc == 0 and np.array_equal(out, out0)
+
+
+def test_plans_used_from_several_threads(oracle):
+    """Plans are independent (each with its own stream and buffers): fields of
+    different frameworks computed on concurrent host threads equal the same
+    fields computed one after another."""
+    from concurrent.futures import ThreadPoolExecutor
+    from string_vext_amd import synthetic
+    from string_vext_amd.field import Plan
+    cases = synthetic.ragged_cases()[:4]
+    _, axes = oracle.regular_pose_grid(8, 64)
+
+    def field(case):
+        plan = Plan.from_case(case)
+        return plan.field(plan.grid(8, axes), precision="mixed")["energy"]
+
+    alone = [field(c) for c in cases]
+    with ThreadPoolExecutor(4) as pool:
+        together = list(pool.map(field, cases))
+    for a, b in zip(alone, together):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("n_axes", [1, 100])
 def test_dropin_abi_axis_counts_off_the_wave_width(oracle, lib, small_case, n_axes):
     """One director, and 100 directors (a full group of 64 lanes and a
