@@ -183,18 +183,21 @@ hipError_t launch_zero(int* p, int n, hipStream_t s);
 // the m[0] x m[1] x m[2] grid and sub the Morton index of its cell of the
 // bin's 4 x 4 x 4 sub-grid, with the lattice axes order & 3, (order >> 2) & 3,
 // (order >> 4) & 3 (longest bin edge first) as the most significant.
+// totals: the packed running totals (runs in the low 32 bits, slots in the
+// high 32) that the bin kernel clears for launch_site_layout.
 hipError_t launch_site_bin(long long n_sites, const double* centers, const double* axes,
                            double o0, double o1, const Mat3& Hinv, const int m[3], int order,
-                           int* cell, int* rank, int* cell_counts, int* bad, hipStream_t s);
-// First slot of every cell (scan of the counts) and the run table: the cells
-// of each bin are taken in order into runs of at most 64 sites (a cell of
-// more than 64 sites gets ceil(count / 64) runs of its own).  The table's
-// length goes to *d_n_runs; it holds at most site_runs_bound(...).
-// d_bin_sites / d_bin_runs: n_bins ints of scratch each.
+                           int* cell, int* rank, int* cell_counts, int* bad,
+                           unsigned long long* totals, hipStream_t s);
+// First slot of every cell and the run table, after launch_site_bin on the
+// same stream: the cells of each bin are taken in order into runs of at most
+// 64 sites (a cell of more than 64 sites gets ceil(count / 64) runs of its
+// own); each bin's slots and runs are contiguous, the bins in no fixed order.
+// The table's length ends in the low 32 bits of *d_totals (the int at its
+// address); it holds at most site_runs_bound(...).
 long long site_runs_bound(int n_bins, long long n_sites);
-hipError_t launch_site_scan(const int* d_cell_counts, int n_bins, int* d_bin_sites,
-                            int* d_bin_runs, int* d_first, SiteRun* d_runs, int* d_n_runs,
-                            hipStream_t s);
+hipError_t launch_site_layout(const int* d_cell_counts, int n_bins, unsigned long long* d_totals,
+                              int* d_first, SiteRun* d_runs, hipStream_t s);
 // Slots of the binned sites (offset from the bin centre, site id) and the
 // cell counts cleared for the next binning.
 hipError_t launch_site_scatter(long long n_sites, const double* centers, const double* axes,

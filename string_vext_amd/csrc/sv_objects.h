@@ -143,8 +143,8 @@ struct PoseGrid {
   DevBuf<int> bin, rank;              // per site: cell of the sub-grid, rank in it
   DevBuf<int> counts, first;          // per cell (64 per bin): sites, first slot
   DevBuf<SiteRun> runs;               // work units of the site kernels
-  DevBuf<int> bin_sites, bin_runs;    // per bin: sites and runs, then their scans
-  DevBuf<int> n_runs;
+  // runs (low 32 bits: the run count once laid out) and slots taken (high 32)
+  DevBuf<unsigned long long> totals;
   DevBuf<double> split2;               // [n_types] class-0 radius^2
   DevBuf<double> site_e, site_g;       // per-site results [2 cap], [2 cap][3]
   bool timing = false;                 // record events around the phases

@@ -538,9 +538,7 @@ int pose_grid_build(Plan* P, const int m_in[3], bool site_lists, PoseGrid** out)
   if (!rc) rc = batch_lists(G->B, false);
   if (!rc && ((e = G->counts.reserve(64 * (size_t)G->n_bins)) != hipSuccess ||
               (e = G->first.reserve(64 * (size_t)G->n_bins)) != hipSuccess ||
-              (e = G->n_runs.reserve(1)) != hipSuccess ||
-              (e = G->bin_sites.reserve(G->n_bins)) != hipSuccess ||
-              (e = G->bin_runs.reserve(G->n_bins)) != hipSuccess ||
+              (e = G->totals.reserve(1)) != hipSuccess ||
               // the binning pass expects cleared cell counts and clears them after use
               (e = hipMemsetAsync(G->counts.p, 0, 64 * sizeof(int) * (size_t)G->n_bins,
                                   G->B->stream)) != hipSuccess ||
@@ -1204,6 +1202,9 @@ Poses::~Poses() { delete list_grid; }
 
 namespace {
 
+// The run count: the low 32 bits of the packed running totals (little endian).
+const int* n_runs_of(const PoseGrid* G) { return reinterpret_cast<const int*>(G->totals.p); }
+
 // Enqueue the site pass and the pose sums on stream st (events around them
 // when the grid times its phases); the sites of the poses are already binned.
 int site_pass(PoseGrid* G, hipStream_t st, int precision, const double* grad_out, double* energy,
@@ -1217,7 +1218,7 @@ int site_pass(PoseGrid* G, hipStream_t st, int precision, const double* grad_out
   const SiteArrays a = {S->d_center_rel.p, S->d_center_id.p, G->site_e.p, G->site_g.p,
                         2LL * G->capacity, G->parts};
   SV_HIP(launch_site(precision, energy != nullptr, grad, B->d_jobs.p, G->runs.p, (int)max_runs,
-                     G->n_runs.p, B->d_offsets.p, B->d_list32.p, B->d_list64.p, a, st));
+                     n_runs_of(G), B->d_offsets.p, B->d_list32.p, B->d_list64.p, a, st));
   if (G->timing) SV_HIP(hipEventRecord(G->ev[2], st));
   SV_HIP(launch_site_combine(n, a, G->P->o0, G->P->o1, grad_out, energy, grad_center, grad_axis,
                              B->d_bad.p, st));
@@ -1265,9 +1266,8 @@ int pose_grid_enqueue(PoseGrid* G, hipStream_t st, int n, const double* centers_
   G->last_stream_set = true;
   if (G->timing) SV_HIP(hipEventRecord(G->ev[0], st));
   SV_HIP(launch_site_bin(n_sites, centers_cart, axes, P->o0, P->o1, G->Hinv, G->m, G->order,
-                         G->bin.p, G->rank.p, G->counts.p, B->d_bad.p, st));
-  SV_HIP(launch_site_scan(G->counts.p, G->n_bins, G->bin_sites.p, G->bin_runs.p, G->first.p,
-                          G->runs.p, G->n_runs.p, st));
+                         G->bin.p, G->rank.p, G->counts.p, B->d_bad.p, G->totals.p, st));
+  SV_HIP(launch_site_layout(G->counts.p, G->n_bins, G->totals.p, G->first.p, G->runs.p, st));
   SV_HIP(launch_site_scatter(n_sites, centers_cart, axes, P->o0, P->o1, G->H, G->Hinv, G->m,
                              G->bin.p, G->rank.p, G->first.p, G->counts.p, S->d_center_rel.p,
                              S->d_center_id.p, st));
@@ -1412,7 +1412,7 @@ int sv_pose_grid_census(void* grid, unsigned long long* pairs) {
   const hipStream_t st = G->last_stream_set ? G->last_stream : B->stream;
   const long long max_runs = site_runs_bound(G->n_bins, 2LL * G->binned_n);
   SV_HIP(launch_zero_u64(B->d_pairs.p, 1, st));
-  SV_HIP(launch_site_census(B->d_jobs.p, G->runs.p, (int)max_runs, G->n_runs.p,
+  SV_HIP(launch_site_census(B->d_jobs.p, G->runs.p, (int)max_runs, n_runs_of(G),
                             B->d_offsets.p, B->d_list64.p, G->S->d_center_rel.p, B->d_pairs.p,
                             st));
   unsigned long long total = 0;

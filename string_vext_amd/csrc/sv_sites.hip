@@ -415,7 +415,8 @@ __device__ __forceinline__ int site_cell_of(const double f[3], int m0, int m1, i
 }
 
 // Cell + rank of every site by one atomic per site; invalid poses go to cell
-// 0 and set flag 2.
+// 0 and set flag 2.  Also clears the running totals of sites and runs that
+// the layout kernel, launched after this one, reserves its ranges from.
 __global__ __launch_bounds__(256) void site_bin_kernel(long long n_sites,
                                                        const double* __restrict__ centers,
                                                        const double* __restrict__ axes, double o0,
@@ -423,8 +424,10 @@ __global__ __launch_bounds__(256) void site_bin_kernel(long long n_sites,
                                                        int order, int* __restrict__ cell,
                                                        int* __restrict__ rank,
                                                        int* __restrict__ cell_counts,
-                                                       int* __restrict__ bad) {
+                                                       int* __restrict__ bad,
+                                                       unsigned long long* __restrict__ totals) {
   const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (k == 0) *totals = 0ull;   // runs (low word) and slots (high word) taken
   if (k >= n_sites) return;
   double f[3];
   int c = 0;
@@ -491,76 +494,58 @@ __device__ __forceinline__ int bin_runs(int c, int b, int first_slot, SiteRun* _
   return n_runs;
 }
 
-// Sites and runs of every bin (one wave per bin).
-__global__ __launch_bounds__(256) void site_bin_totals_kernel(const int* __restrict__ cell_counts,
-                                                              int n_bins,
-                                                              int* __restrict__ bin_sites,
-                                                              int* __restrict__ bin_runs_n) {
-  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (b >= n_bins) return;
-  const int lane = threadIdx.x & 63;
-  const int c = cell_counts[64 * (long long)b + lane];
-  int total = c;
+// Slots and runs of every bin (one wave per bin, kLayoutWaves bins per
+// block).  A block reserves the range of slots and the range of the run
+// table for its bins with one 64-bit atomic on the packed running totals
+// (runs in the low word, slots in the high word; cleared by the binning
+// kernel), so the blocks take their ranges in no fixed order while the bins
+// of a block stay in bin order; each bin's slots and runs are contiguous.  No
+// result depends on that order: a site's sums depend only on the members of
+// its run (which fix the run's box and culled list), and they are gathered
+// by site index.  The low word ends as the length of the run table.
+constexpr int kLayoutWaves = 16;
+
+__global__ __launch_bounds__(64 * kLayoutWaves) void site_layout_kernel(
+    const int* __restrict__ cell_counts, int n_bins, unsigned long long* __restrict__ totals,
+    int* __restrict__ first, SiteRun* __restrict__ runs) {
+  __shared__ int s_sites[kLayoutWaves], s_runs[kLayoutWaves];
+  __shared__ unsigned long long s_base;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int b = blockIdx.x * kLayoutWaves + wv;
+  int c = 0, incl = 0, total = 0, k = 0;
+  if (b < n_bins) {
+    c = cell_counts[64 * (long long)b + lane];
+    incl = c;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
-  const int k = bin_runs(c, b, 0, nullptr);
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    total = __shfl(incl, 63, 64);
+    if (total > 0) k = bin_runs(c, b, 0, nullptr);
+  }
   if (lane == 0) {
-    bin_sites[b] = total;
-    bin_runs_n[b] = k;
+    s_sites[wv] = total;
+    s_runs[wv] = k;
   }
-}
-
-// One block: exclusive scans of the bins' sites and runs, in place; the run
-// total goes to *n_runs.
-__global__ __launch_bounds__(1024) void site_bin_scan_kernel(int n_bins, int* __restrict__ bin_sites,
-                                                             int* __restrict__ bin_runs_n,
-                                                             int* __restrict__ n_runs) {
-  __shared__ int sa[1024], sb[1024];
-  const int t = threadIdx.x;
-  int carry_a = 0, carry_b = 0;
-  for (int base = 0; base < n_bins; base += 1024) {
-    const int b = base + t;
-    const int va0 = b < n_bins ? bin_sites[b] : 0, vb0 = b < n_bins ? bin_runs_n[b] : 0;
-    sa[t] = va0;
-    sb[t] = vb0;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int va = t >= off ? sa[t - off] : 0, vb = t >= off ? sb[t - off] : 0;
-      __syncthreads();
-      sa[t] += va;
-      sb[t] += vb;
-      __syncthreads();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sites = 0, nr = 0;
+    for (int w = 0; w < kLayoutWaves; ++w) {
+      sites += (unsigned)s_sites[w];
+      nr += (unsigned)s_runs[w];
     }
-    if (b < n_bins) {
-      bin_sites[b] = carry_a + sa[t] - va0;
-      bin_runs_n[b] = carry_b + sb[t] - vb0;
-    }
-    carry_a += sa[1023];
-    carry_b += sb[1023];
-    __syncthreads();
+    s_base = (sites | nr) ? atomicAdd(totals, (sites << 32) | nr) : 0ull;
   }
-  if (t == 0) *n_runs = carry_b;
-}
-
-// First slot of every cell and the run table (one wave per bin).
-__global__ __launch_bounds__(256) void site_runs_kernel(const int* __restrict__ cell_counts,
-                                                        int n_bins,
-                                                        const int* __restrict__ bin_first,
-                                                        const int* __restrict__ bin_run0,
-                                                        int* __restrict__ first,
-                                                        SiteRun* __restrict__ runs) {
-  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (b >= n_bins) return;
-  const int lane = threadIdx.x & 63;
-  const int c = cell_counts[64 * (long long)b + lane];
-  int incl = c;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += y;
+  __syncthreads();
+  if (b >= n_bins || total == 0) return;   // wave-uniform
+  int site0 = (int)(s_base >> 32), run0 = (int)(s_base & 0xffffffffull);
+  for (int w = 0; w < wv; ++w) {
+    site0 += s_sites[w];
+    run0 += s_runs[w];
   }
-  first[64 * (long long)b + lane] = bin_first[b] + incl - c;
-  bin_runs(c, b, bin_first[b], runs + bin_run0[b]);
+  first[64 * (long long)b + lane] = site0 + incl - c;
+  bin_runs(c, b, site0, runs + run0);
 }
 
 // Tile-sorted slots: Cartesian offset of the site from its bin centre, from
@@ -625,11 +610,12 @@ __global__ __launch_bounds__(64) void site_census_kernel(const Job* __restrict__
 
 hipError_t launch_site_bin(long long n_sites, const double* centers, const double* axes,
                            double o0, double o1, const Mat3& Hinv, const int m[3], int order,
-                           int* cell, int* rank, int* cell_counts, int* bad, hipStream_t s) {
+                           int* cell, int* rank, int* cell_counts, int* bad,
+                           unsigned long long* totals, hipStream_t s) {
   if (n_sites <= 0) return hipSuccess;
   hipLaunchKernelGGL(site_bin_kernel, dim3((unsigned)((n_sites + 255) / 256)), dim3(256), 0, s,
                      n_sites, centers, axes, o0, o1, Hinv, m[0], m[1], m[2], order, cell, rank,
-                     cell_counts, bad);
+                     cell_counts, bad, totals);
   return hipGetLastError();
 }
 
@@ -643,17 +629,12 @@ long long site_runs_bound(int n_bins, long long n_sites) {
   return (long long)n_bins + 3 * ((n_sites + 63) / 64);
 }
 
-hipError_t launch_site_scan(const int* d_cell_counts, int n_bins, int* d_bin_sites,
-                            int* d_bin_runs, int* d_first, SiteRun* d_runs, int* d_n_runs,
-                            hipStream_t s) {
+hipError_t launch_site_layout(const int* d_cell_counts, int n_bins, unsigned long long* d_totals,
+                              int* d_first, SiteRun* d_runs, hipStream_t s) {
   if (n_bins <= 0) return hipSuccess;
-  const dim3 grid((unsigned)((n_bins + 3) / 4));
-  hipLaunchKernelGGL(site_bin_totals_kernel, grid, dim3(256), 0, s, d_cell_counts, n_bins,
-                     d_bin_sites, d_bin_runs);
-  hipLaunchKernelGGL(site_bin_scan_kernel, dim3(1), dim3(1024), 0, s, n_bins, d_bin_sites,
-                     d_bin_runs, d_n_runs);
-  hipLaunchKernelGGL(site_runs_kernel, grid, dim3(256), 0, s, d_cell_counts, n_bins, d_bin_sites,
-                     d_bin_runs, d_first, d_runs);
+  hipLaunchKernelGGL(site_layout_kernel, dim3((unsigned)((n_bins + kLayoutWaves - 1) / kLayoutWaves)),
+                     dim3(64 * kLayoutWaves), 0, s, d_cell_counts, n_bins, d_totals, d_first,
+                     d_runs);
   return hipGetLastError();
 }
 
