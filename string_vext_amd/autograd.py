@@ -63,12 +63,13 @@ class PoseEnergy(torch.autograd.Function):
         token = object()
         ctx.fused = bool(fused) and any(ctx.needs_input_grad[:2])
         if ctx.fused:
-            grad_c = torch.empty((len(c), 3), dtype=torch.float64, device=device)
-            grad_u = torch.empty((len(c), 3), dtype=torch.float64, device=device)
+            # dE/dc and dE/du side by side, so that the backward scales both
+            # with one multiplication
+            grads = torch.empty((2, len(c), 3), dtype=torch.float64, device=device)
             grid.eval_async(stream, len(c), c.data_ptr(), u.data_ptr(), precision,
-                            energy=energy.data_ptr(), grad_center=grad_c.data_ptr(),
-                            grad_axis=grad_u.data_ptr(), token=token)
-            ctx.save_for_backward(grad_c, grad_u)
+                            energy=energy.data_ptr(), grad_center=grads[0].data_ptr(),
+                            grad_axis=grads[1].data_ptr(), token=token)
+            ctx.save_for_backward(grads)
         else:
             grid.eval_async(stream, len(c), c.data_ptr(), u.data_ptr(), precision,
                             energy=energy.data_ptr(), token=token)
@@ -83,9 +84,10 @@ class PoseEnergy(torch.autograd.Function):
     @once_differentiable   # the kernels give first derivatives only: a second raises
     def backward(ctx, grad_energy):
         if ctx.fused:
-            grad_c, grad_u = ctx.saved_tensors
-            g = grad_energy.detach().to(device=grad_c.device, dtype=torch.float64)[:, None]
-            return (g * grad_c).to(ctx.dtypes[0]), (g * grad_u).to(ctx.dtypes[1]), None, None, None
+            (grads,) = ctx.saved_tensors
+            g = grad_energy.detach().to(device=grads.device, dtype=torch.float64)
+            scaled = grads * g[None, :, None]
+            return scaled[0].to(ctx.dtypes[0]), scaled[1].to(ctx.dtypes[1]), None, None, None
         c, u = ctx.saved_tensors
         n = len(c)
         g = grad_energy.detach().to(device=c.device, dtype=torch.float64).contiguous()
