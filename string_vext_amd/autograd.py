@@ -4,7 +4,8 @@
     E.sum().backward()                                        # centers.grad, axes.grad
 
 ``centers`` are Cartesian (Angstrom), ``axes`` the directors, both (n, 3)
-CUDA tensors; directors are used as given (normalize them in torch if they are
+tensors, normally on the GPU (host tensors are copied there, and their
+energies and gradients come back to the host); directors are used as given (normalize them in torch if they are
 free parameters, and autograd carries the normalization).  The poses stay on
 the device: each call bins them into the plan's cached tile grid on the GPU,
 enqueued on torch's current stream without a host synchronization
@@ -78,7 +79,9 @@ class PoseEnergy(torch.autograd.Function):
         ctx.token = token
         ctx.precision = precision
         ctx.dtypes = (centers.dtype, axes.dtype)
-        return energy
+        ctx.devices = (centers.device, axes.device)
+        # host inputs get host energies (and host gradients in the backward)
+        return energy if centers.is_cuda else energy.to(centers.device)
 
     @staticmethod
     @once_differentiable   # the kernels give first derivatives only: a second raises
@@ -87,7 +90,8 @@ class PoseEnergy(torch.autograd.Function):
             (grads,) = ctx.saved_tensors
             g = grad_energy.detach().to(device=grads.device, dtype=torch.float64)
             scaled = grads * g[None, :, None]
-            return scaled[0].to(ctx.dtypes[0]), scaled[1].to(ctx.dtypes[1]), None, None, None
+            return (scaled[0].to(device=ctx.devices[0], dtype=ctx.dtypes[0]),
+                    scaled[1].to(device=ctx.devices[1], dtype=ctx.dtypes[1]), None, None, None)
         c, u = ctx.saved_tensors
         n = len(c)
         g = grad_energy.detach().to(device=c.device, dtype=torch.float64).contiguous()
@@ -103,7 +107,8 @@ class PoseEnergy(torch.autograd.Function):
             ctx.grid.eval_async(stream, n, c.data_ptr(), u.data_ptr(), ctx.precision,
                                 grad_out=g.data_ptr(), grad_center=grad_c.data_ptr(),
                                 grad_axis=grad_u.data_ptr())
-        return grad_c.to(ctx.dtypes[0]), grad_u.to(ctx.dtypes[1]), None, None, None
+        return (grad_c.to(device=ctx.devices[0], dtype=ctx.dtypes[0]),
+                grad_u.to(device=ctx.devices[1], dtype=ctx.dtypes[1]), None, None, None)
 
 
 def check_poses(plan: Plan) -> None:

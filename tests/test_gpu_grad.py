@@ -419,3 +419,24 @@ def test_second_derivative_is_refused(case, poses, fused):
     assert torch.isfinite(g).all()
     with pytest.raises(RuntimeError, match="once_differentiable"):
         g.sum().backward()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_host_tensors_get_host_energies_and_gradients(case, poses, fused):
+    """Host (CPU) inputs are evaluated on the GPU; their energies and gradients
+    come back on the host, equal to those of the same inputs on the GPU."""
+    import torch
+    from string_vext_amd.autograd import pose_energy
+    centers, axes = poses
+    plan = _plan(case)
+    out = {}
+    for device in ("cpu", "cuda"):
+        c = torch.tensor(centers[:200], dtype=torch.float64, device=device, requires_grad=True)
+        u = torch.tensor(axes[:200], dtype=torch.float64, device=device, requires_grad=True)
+        e = pose_energy(plan, c, u, "mixed", fused=fused)
+        assert e.device.type == device
+        e.sum().backward()
+        assert c.grad.device.type == device and u.grad.device.type == device
+        out[device] = [t.detach().cpu() for t in (e, c.grad, u.grad)]
+    for a, b in zip(out["cpu"], out["cuda"]):
+        assert torch.equal(a, b)
