@@ -113,9 +113,31 @@ constexpr int kStreamTypes = 16;
 // [0, n_poses) take x0, waves [n_poses, 2 n_poses) take x2; results in Vd.
 struct Recheck {
   const double* x2;
+  const double* t2;   // trig table of x2
   const double* V0;
   const double* V2;
 };
+
+// Trig table of positions x [P][6]: (sin beta, cos beta, sin gamma, cos
+// gamma) per image [P][4], written wherever the positions are (string
+// creation, the step kernel for x2, the finish kernel for x0), so that the
+// image kernels, whose waves are latency-bound, do not each evaluate two FP64
+// sincos at their start.
+__device__ __forceinline__ void image_trig(const double* __restrict__ x6, double* __restrict__ t4) {
+  double sb, cb, sg, cg;
+  sincos(x6[4], &sb, &cb);
+  sincos(x6[5], &sg, &cg);
+  t4[0] = sb;
+  t4[1] = cb;
+  t4[2] = sg;
+  t4[3] = cg;
+}
+
+__global__ __launch_bounds__(256) void string_trig_kernel(int P, const double* __restrict__ x,
+                                                          double* __restrict__ t) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p < P) image_trig(x + 6LL * p, t + 4LL * p);
+}
 
 // Energy (and gradient) of one pose per wave.  x = (a, b, c, alpha, beta,
 // gamma) with fractional center (a, b, c).  The image is located in its bin;
@@ -126,6 +148,7 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
                                                     const int* __restrict__ plan_of, int n_images,
                                                     int n_poses, const int* __restrict__ converged,
                                                     const double* __restrict__ x_in,
+                                                    const double* __restrict__ trig_in,
                                                     double* __restrict__ V, double* __restrict__ G,
                                                     Recheck rc = Recheck{}) {
   __shared__ int s_near[4][128];   // queued near-contact candidates per wave
@@ -142,11 +165,13 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
   const int slot = blockIdx.x * 4 + wv;   // output index
   int p = slot;                           // image index
   const double* __restrict__ x = x_in;
+  const double* __restrict__ trig = trig_in;
   if (RECHECK) {
     if (slot >= 2 * n_poses) return;
     if (slot >= n_poses) {
       p = slot - n_poses;
       x = rc.x2;
+      trig = rc.t2;
     }
     if (!decision_in_doubt(rc.V0[p], rc.V2[p])) return;
   } else if (p >= n_poses) {
@@ -156,10 +181,8 @@ __global__ __launch_bounds__(256) void image_kernel(const ImageGrid* __restrict_
   if (converged && converged[s]) return;
   const ImageGrid& IG = grids[plan_of[s]];
   const Job& J = *IG.job;
-  const double be = x[6 * p + 4], ga = x[6 * p + 5];
-  double sb, cb, sg, cg;
-  sincos(be, &sb, &cb);
-  sincos(ga, &sg, &cg);
+  const double sb = ld_scalar(trig, 4LL * p), cb = ld_scalar(trig, 4LL * p + 1),
+               sg = ld_scalar(trig, 4LL * p + 2), cg = ld_scalar(trig, 4LL * p + 3);
   // the image's director and center are wave-uniform: scalar registers
   const double u[3] = {uniform_d(cg * cb), uniform_d(sg * cb), uniform_d(-sb)};
   const float uf[3] = {(float)u[0], (float)u[1], (float)u[2]};
@@ -450,7 +473,8 @@ __global__ __launch_bounds__(256) void string_step_kernel(const ImageGrid* __res
                                                           const int* __restrict__ converged,
                                                           const double* __restrict__ x0,
                                                           const double* __restrict__ G,
-                                                          double* __restrict__ x2) {
+                                                          double* __restrict__ x2,
+                                                          double* __restrict__ t2) {
   extern __shared__ double sh[];
   const int s = blockIdx.x;
   if (converged[s]) return;
@@ -557,6 +581,7 @@ __global__ __launch_bounds__(256) void string_step_kernel(const ImageGrid* __res
       x2[6 * p + k] = (f1 > f0) ? a0 + (target - f0) * ((a1 - a0) / (f1 - f0)) : a0;
     }
     for (int k = 3; k < 6; ++k) x2[6 * p + k] = s1[6 * i + k];
+    image_trig(s1 + 6 * i, t2 + 4LL * p);
   }
 }
 
@@ -580,6 +605,7 @@ __global__ __launch_bounds__(256) void string_finish_kernel(StringParams sp,
                                                             int* __restrict__ converged,
                                                             int* __restrict__ iterations,
                                                             double* __restrict__ x0,
+                                                            double* __restrict__ t0,
                                                             const double* __restrict__ x2,
                                                             const double* __restrict__ V0,
                                                             const double* __restrict__ V2,
@@ -640,31 +666,43 @@ __global__ __launch_bounds__(256) void string_finish_kernel(StringParams sp,
       else if (a < 0.0) a += 2.0 * kPi;
       x0[6 * p + 3 + k] = a;
     }
+    image_trig(x0 + 6LL * p, t0 + 4LL * p);
   }
   if (threadIdx.x == 0) iterations[s] = it + 1;
 }
 
+// x: positions [P][6]; t: their trig table [P][4] (image_trig).
 hipError_t launch_image(int precision, bool grad, const ImageGrid* plans, const int* plan_of, int N,
-                        int P, const int* converged, const double* x, double* V, double* G,
-                        hipStream_t st) {
+                        int P, const int* converged, const double* x, const double* t, double* V,
+                        double* G, hipStream_t st) {
   const dim3 g((P + 3) / 4), b(256);
+#define SV_IMAGE(M, GR)                                                                        \
+  hipLaunchKernelGGL((image_kernel<M, GR>), g, b, 0, st, plans, plan_of, N, P, converged, x, t, \
+                     V, G, Recheck{})
   if (precision == PREC_MIXED) {
-    if (grad) hipLaunchKernelGGL((image_kernel<true, true>), g, b, 0, st, plans, plan_of, N, P, converged, x, V, G);
-    else hipLaunchKernelGGL((image_kernel<true, false>), g, b, 0, st, plans, plan_of, N, P, converged, x, V, G);
+    if (grad) SV_IMAGE(true, true);
+    else SV_IMAGE(true, false);
   } else {
-    if (grad) hipLaunchKernelGGL((image_kernel<false, true>), g, b, 0, st, plans, plan_of, N, P, converged, x, V, G);
-    else hipLaunchKernelGGL((image_kernel<false, false>), g, b, 0, st, plans, plan_of, N, P, converged, x, V, G);
+    if (grad) SV_IMAGE(false, true);
+    else SV_IMAGE(false, false);
   }
+#undef SV_IMAGE
   return hipGetLastError();
 }
 
 // FP64 energies of the images of a mixed run whose revert decision is in doubt.
 hipError_t launch_recheck(const ImageGrid* plans, const int* plan_of, int N, int P,
-                          const int* converged, const double* x0, const double* x2,
-                          const double* V0, const double* V2, double* Vd, hipStream_t st) {
+                          const int* converged, const double* x0, const double* t0,
+                          const double* x2, const double* t2, const double* V0, const double* V2,
+                          double* Vd, hipStream_t st) {
   const dim3 g((2 * P + 3) / 4), b(256);
   hipLaunchKernelGGL((image_kernel<false, false, true>), g, b, 0, st, plans, plan_of, N, P,
-                     converged, x0, Vd, nullptr, Recheck{x2, V0, V2});
+                     converged, x0, t0, Vd, nullptr, Recheck{x2, t2, V0, V2});
+  return hipGetLastError();
+}
+
+hipError_t launch_trig(int P, const double* x, double* t, hipStream_t st) {
+  hipLaunchKernelGGL(string_trig_kernel, dim3((P + 255) / 256), dim3(256), 0, st, P, x, t);
   return hipGetLastError();
 }
 
@@ -679,6 +717,7 @@ struct StringSys {
   DevBuf<ImageGrid> d_grids;
   DevBuf<int> d_plan_of, d_dir, d_conv, d_iter;
   DevBuf<double> d_x0, d_x2, d_G, d_V0, d_V2, d_Vd;   // d_Vd: mixed runs' FP64 rechecks [2P]
+  DevBuf<double> d_t0, d_t2;                          // trig tables of x0 and x2 [P][4]
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~StringSys() {
@@ -702,19 +741,19 @@ int iteration(StringSys* Y) {
   const size_t step_shm = sizeof(double) * 7 * Y->N;
   const size_t fin_shm = sizeof(double) * (3 * Y->N + 2 * 256);
   SV_HIP_S(launch_image(Y->precision, true, Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
-                        Y->d_x0.p, Y->d_V0.p, Y->d_G.p, st));
+                        Y->d_x0.p, Y->d_t0.p, Y->d_V0.p, Y->d_G.p, st));
   hipLaunchKernelGGL(string_step_kernel, dim3(Y->S), dim3(256), step_shm, st, Y->d_grids.p,
                      Y->d_plan_of.p, Y->d_dir.p, Y->sp, Y->d_conv.p, Y->d_x0.p, Y->d_G.p,
-                     Y->d_x2.p);
+                     Y->d_x2.p, Y->d_t2.p);
   SV_HIP_S(hipGetLastError());
   SV_HIP_S(launch_image(Y->precision, false, Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p,
-                        Y->d_x2.p, Y->d_V2.p, nullptr, st));
+                        Y->d_x2.p, Y->d_t2.p, Y->d_V2.p, nullptr, st));
   const bool mixed = Y->precision == PREC_MIXED;
   if (mixed)
     SV_HIP_S(launch_recheck(Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, Y->d_conv.p, Y->d_x0.p,
-                            Y->d_x2.p, Y->d_V0.p, Y->d_V2.p, Y->d_Vd.p, st));
+                            Y->d_t0.p, Y->d_x2.p, Y->d_t2.p, Y->d_V0.p, Y->d_V2.p, Y->d_Vd.p, st));
   hipLaunchKernelGGL(string_finish_kernel, dim3(Y->S), dim3(256), fin_shm, st, Y->sp, Y->d_conv.p,
-                     Y->d_iter.p, Y->d_x0.p, Y->d_x2.p, Y->d_V0.p, Y->d_V2.p,
+                     Y->d_iter.p, Y->d_x0.p, Y->d_t0.p, Y->d_x2.p, Y->d_V0.p, Y->d_V2.p,
                      mixed ? Y->d_Vd.p : nullptr);
   SV_HIP_S(hipGetLastError());
   return 0;
@@ -797,6 +836,9 @@ int sv_strings_create(int n_strings, void* const* plans, int n_images, const dou
       (e = Y->d_conv.upload(zeros.data(), n_strings, Y->stream)) != hipSuccess ||
       (e = Y->d_iter.upload(zeros.data(), n_strings, Y->stream)) != hipSuccess ||
       (e = Y->d_x0.upload(init, 6 * (size_t)Y->P, Y->stream)) != hipSuccess ||
+      (e = Y->d_t0.reserve(4 * (size_t)Y->P)) != hipSuccess ||
+      (e = Y->d_t2.reserve(4 * (size_t)Y->P)) != hipSuccess ||
+      (e = launch_trig(Y->P, Y->d_x0.p, Y->d_t0.p, Y->stream)) != hipSuccess ||
       (e = Y->d_x2.reserve(6 * (size_t)Y->P)) != hipSuccess ||
       (e = Y->d_G.reserve(6 * (size_t)Y->P)) != hipSuccess ||
       (e = Y->d_V0.reserve(Y->P)) != hipSuccess || (e = Y->d_V2.reserve(Y->P)) != hipSuccess ||
@@ -844,7 +886,7 @@ int sv_strings_state(void* strings, double* path, double* energy, int* iteration
   SV_HIP_S(hipSetDevice(Y->device));
   if (energy)
     SV_HIP_S(launch_image(Y->precision, false, Y->d_grids.p, Y->d_plan_of.p, Y->N, Y->P, nullptr,
-                          Y->d_x0.p, Y->d_V0.p, nullptr, Y->stream));
+                          Y->d_x0.p, Y->d_t0.p, Y->d_V0.p, nullptr, Y->stream));
   if (path) SV_HIP_S(hipMemcpyAsync(path, Y->d_x0.p, sizeof(double) * 6 * Y->P, hipMemcpyDeviceToHost, Y->stream));
   if (energy) SV_HIP_S(hipMemcpyAsync(energy, Y->d_V0.p, sizeof(double) * Y->P, hipMemcpyDeviceToHost, Y->stream));
   if (iterations) SV_HIP_S(hipMemcpyAsync(iterations, Y->d_iter.p, sizeof(int) * Y->S, hipMemcpyDeviceToHost, Y->stream));
