@@ -73,3 +73,27 @@ def test_pose_gradient_has_no_director_part(oracle, case, precision):
     g = c.grad.cpu().numpy()
     gs = np.maximum(np.abs(gc_ref).max(axis=1, keepdims=True), 1.0)
     assert (np.abs(g - gc_ref) / gs)[acc].max() < (1e-9 if precision == "fp64" else 1e-4)
+
+
+def test_string_iterations_match_cpu_restatement(oracle, case):
+    """Strings of the coincident-site guest: no angle force (dV/d angle = 0),
+    and iterations equal the CPU restatement."""
+    from oracle import string_oracle as SO
+    from string_vext_amd.field import Plan
+    from string_vext_amd.strings import StringSystem, straight_strings
+    plan = Plan.from_case(case)
+    x0 = np.concatenate([straight_strings(plan, d, 1, 12, seed=70 + d) for d in (1, 2, 3)])
+    dirs = np.array([1, 2, 3])
+    setup = oracle.ReuseSetup(case)
+    system = StringSystem(plan, x0, dirs, precision="fp64")
+    system.run(3, graph=True)
+    ref = x0.copy()
+    for it in range(3):
+        ref = np.stack([SO.iterate(case, ref[s], int(dirs[s]), it, setup=setup)[0]
+                        for s in range(len(ref))])
+    path, energy, its, conv = system.state()
+    assert (its == 3).all()
+    np.testing.assert_allclose(path[..., :3], ref[..., :3], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(path[..., 3:], ref[..., 3:], rtol=0, atol=1e-9)
+    _, g = SO.energy_grad(case, ref[0], setup)
+    assert np.abs(g[:, 3:]).max() == 0.0
