@@ -143,16 +143,22 @@ class _Digester:
 
     def __init__(self, digest):
         self.digest = digest
+        self.error = None
         self.blocks = queue.Queue(maxsize=256)
         self.thread = threading.Thread(target=self._run, daemon=True)
         self.thread.start()
 
     def _run(self):
+        # keeps draining after a failed update, so that feed never blocks for good
         while True:
             block = self.blocks.get()
             if block is None:
                 return
-            self.digest.update(block)
+            if self.error is None:
+                try:
+                    self.digest.update(block)
+                except Exception as exc:   # reported by close()
+                    self.error = exc
 
     def feed(self, block: bytes):
         self.blocks.put(block)
@@ -160,6 +166,8 @@ class _Digester:
     def close(self):
         self.blocks.put(None)
         self.thread.join()
+        if self.error is not None:
+            raise RuntimeError("hashing the written archive failed") from self.error
 
 
 def _dos_time(t: float):

@@ -137,3 +137,14 @@ def test_digest_is_the_hash_of_the_written_bytes(monkeypatch):
         npz.write_npz(buf, fields, threads=4, digest=h)
         assert h.hexdigest() == hashlib.sha256(buf.getvalue()).hexdigest()
         _check(buf, fields)
+
+
+def test_failing_digest_is_reported_not_hung():
+    """A digest whose update raises ends the write with an error (the hashing
+    thread keeps draining, so the writer never waits on it for good)."""
+    class Broken:
+        def update(self, block):
+            raise ValueError("broken digest")
+    fields = {"a": np.arange(300_000, dtype=np.float64)}
+    with pytest.raises(RuntimeError, match="hashing"):
+        npz.write_npz(io.BytesIO(), fields, threads=2, digest=Broken())
