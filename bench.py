@@ -681,7 +681,7 @@ def field_line(args):
     if args.tile_centers > 0:
         tile_centers = args.tile_centers
     else:
-        tuned = autotune.tune([case], size=SIZE, orientations=ORIENTATIONS, reps=2, persistent=False)
+        tuned = autotune.tune([case], size=SIZE, orientations=ORIENTATIONS, reps=5, persistent=False)
         tile_centers = tuned["tile_centers"][autotune.density_class(case)]
         log(f"autotune: tile_centers={tile_centers} trials={tuned['trials']}")
     plan = Plan.from_case(case)

@@ -38,17 +38,19 @@ def density_class(case) -> str:
 
 
 def _time_batch(jobs, precision, blocks_per_cu, reps):
+    """Median over `reps` runs of the whole device-side run (HIP events around
+    it: list count, scan and fill, the field kernel, the epilogue and the gaps
+    between them), the cost a step pays for the tile size."""
     batch = Batch(jobs, precision=precision, outputs=("orientation",))
     if blocks_per_cu:
         batch.set_persistent(blocks_per_cu)
     batch.run(sync=True)
-    best = float("inf")
+    runs = []
     for _ in range(reps):
         batch.run(time_kernels=True, sync=True)
-        t = batch.timings()
-        best = min(best, t["list_count"] + t["scan"] + t["list_fill"] + t["field"])
+        runs.append(batch.timings()["run"])
     batch.close()
-    return best
+    return float(np.median(runs))
 
 
 def tune(cases, size=24, orientations=64, precision="mixed", reps=3, use_cache=True,
