@@ -258,7 +258,7 @@ def ragged_line(args):
     from string_vext_amd.orientation import regular_pose_grid
     t0 = time.perf_counter()
     cases = synthetic.ragged_cases()
-    tuned = autotune.tune(cases)
+    tuned = autotune.tune(cases, size=SIZE)   # trials on the fields' own 60^3 grid
     log(f"autotune: {tuned['tile_centers']} blocks/CU {tuned['blocks_per_cu']} "
         f"({time.perf_counter() - t0:.1f} s)")
     _, axes = regular_pose_grid(SIZE, ORIENTATIONS)
