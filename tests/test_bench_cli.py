@@ -44,3 +44,16 @@ def test_error_metrics_definitions():
     assert m["all_max_relative_error_floor1K"] == pytest.approx(2e-3)
     # 100 K floor: 1 / 2500 = 4e-4 is the largest
     assert m["all_max_relative_error_floor100K"] == pytest.approx(4e-4)
+
+
+def test_traffic_table_covers_every_tile_candidate():
+    """The bench line's roofline.traffic comes from the PMC table of the tile
+    size the autotuner picked: every candidate has an entry, each read + write
+    and above the algorithmic bytes."""
+    import json
+    from string_vext_amd import autotune
+    table = json.loads((ROOT / "profiles" / "field_mixed_60x64_traffic.json").read_text())
+    for tc in autotune.TILE_CANDIDATES:
+        rec = table["by_tile"][str(tc)]
+        assert rec["hbm_bytes_per_launch"] == rec["read_bytes"] + rec["write_bytes"]
+        assert rec["hbm_bytes_per_launch"] > table["algorithmic_write_bytes"]
